@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Benchmark: 512^3 dealiased u x omega fields/s on one MI355X.
+
+One step = one dealiased rotational nonlinear term N^ = Q M P F(u x w) of a
+512^3 Float64 solenoidal velocity (seeded k^-5/3 spectrum), computed by the
+fused pass schedule of libgbg with the input already resident in HBM.
+Prints ONE JSON line (rank 0).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512]
+                  [--dtype f64|f32] [--no-cpu-baseline] [--no-fp32]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    name = "gbg_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg_dir = os.path.join(REPO, "gbgperiodicfields.jl_amd")
+    spec = importlib.util.spec_from_file_location(
+        name, os.path.join(pkg_dir, "__init__.py"), submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class HipEvents:
+    """hipEvent timing on the stream the kernels are launched on."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e.value
+
+    def record(self, ev, stream):
+        assert self.hip.hipEventRecord(ev, stream) == 0
+
+    def elapsed(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def uxw_bytes(n, elem):
+    """Algorithmic bytes of one u x omega term: 42 component-sized transfers of
+    M = (Nx/2+1) Ny Nz complex elements (the fused five-pass ideal)."""
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    return 42 * M * 2 * elem
+
+
+def stage_bytes(n, elem):
+    """Algorithmic bytes of each pass of the fused schedule, in component
+    transfers (reads + writes) of M complex elements."""
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    comp = M * 2 * elem
+    return {"K1_z_inverse_curl": 9 * comp, "K2_y_inverse": 12 * comp,
+            "K3_x_product": 9 * comp, "K4_y_forward": 6 * comp,
+            "K5_z_forward_project": 6 * comp}
+
+
+def run_gpu(G, n, dtype, steps, warmup, stream, ev, seed=2001):
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    t0 = time.time()
+    from gbg_amd import workload  # noqa: E402
+    u = workload.synthetic_field(n, seed, vector=True)
+    gen_s = time.time() - t0
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    del u
+    dn = G.Field(grid, "spectral", 1, dtype)
+    kc = G.dealias_cutoff(n[0])
+    for _ in range(warmup):
+        G.uxw_(dn, du, plans, kc, reductions=False, stream=stream)
+    stream.synchronize()
+    evs = [ev.create() for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev.record(evs[0], stream.handle)
+    for i in range(steps):
+        G.uxw_(dn, du, plans, kc, reductions=False, stream=stream)
+        ev.record(evs[i + 1], stream.handle)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    per = [ev.elapsed(evs[i], evs[i + 1]) for i in range(steps)]
+    total_ms = ev.elapsed(evs[0], evs[-1])
+    # per-pass durations, measured with events on the same stream
+    plans.set_profiling(True)
+    for _ in range(min(steps, 16)):
+        G.uxw_(dn, du, plans, kc, reductions=False, stream=stream)
+    stream.synchronize()
+    stages = plans.stage_times()
+    plans.set_profiling(False)
+    E, Z = G.uxw_(dn, du, plans, kc, reductions=True, stream=stream)
+    return dict(plans=plans, du=du, dn=dn, per=per, total_ms=total_ms, wall=wall, stages=stages,
+                E=E, Z=Z, gen_s=gen_s, kc=kc)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the spectral path here runs on a single MI355X and has "
+              "no multi-GPU form (see DESIGN.md for how it would shard). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    G = load_package()
+    if G.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    n = (args.size,) * 3
+    dtype = np.float64 if args.dtype == "f64" else np.float32
+    elem = np.dtype(dtype).itemsize
+    ev = HipEvents()
+    stream = G.Stream()
+    print(f"[bench] {n} {args.dtype} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+
+    r = run_gpu(G, n, dtype, args.steps, args.warmup, stream, ev)
+    ms = r["total_ms"] / args.steps
+    fields_s = 1000.0 / ms
+    B = uxw_bytes(n, elem)
+    per = sorted(r["per"])
+    p50 = per[len(per) // 2]
+    p99 = per[min(len(per) - 1, int(math.ceil(0.99 * len(per))) - 1)]
+    print(f"[bench] gpu {ms:.3f} ms/step, stages {r['stages']}", file=sys.stderr, flush=True)
+
+    # dominant pass and its roofline
+    names = list(stage_bytes(n, elem).keys())
+    sb = stage_bytes(n, elem)
+    stages = r["stages"]
+    roof = None
+    if len(stages) == len(names):
+        i = int(np.argmax(stages))
+        ach = sb[names[i]] / (stages[i] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": names[i], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_ms": round(stages[i], 4)}
+    pipeline_gbs = B / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "512^3 dealiased u x omega fields/s (FP64) + % HBM roofline; FP32 vs FP64; p99 latency",
+        "value": round(fields_s, 3),
+        "unit": "fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if dtype == np.float64 else "f32",
+        "data": "synthetic (seeded k^-5/3 solenoidal spectrum, seed 2001)",
+        "config": {"workload": f"dealiased u x omega, {n[0]}x{n[1]}x{n[2]} vector field, 2/3 rule, "
+                               "Leray projection, energy/enstrophy", "grid": list(n),
+                   "kc": r["kc"], "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
+        "pipeline_roofline": {"bytes_alg": B, "achieved": round(pipeline_gbs, 1),
+                              "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
+        "stage_ms": [round(s, 4) for s in stages],
+        "roofline": roof,
+    }
+    nh_gpu = r["dn"].download()
+    E, Z = r["E"], r["Z"]
+    del r
+
+    if dtype == np.float64 and not args.no_fp32:
+        r32 = run_gpu(G, n, np.float32, args.steps, args.warmup, stream, ev)
+        ms32 = r32["total_ms"] / args.steps
+        n32 = r32["dn"].download()
+        out["fp32"] = {"value": round(1000.0 / ms32, 3), "ms_per_step": round(ms32, 4),
+                       "speedup_vs_fp64": round(ms / ms32, 3),
+                       "stage_ms": [round(s, 4) for s in r32["stages"]]}
+        del r32
+    else:
+        n32 = None
+
+    if not args.no_cpu_baseline:
+        os.environ["GBG_ORACLE_WORKERS"] = str(args.cpu_workers)
+        sys.path.insert(0, REPO)
+        from oracle import gbg_oracle as O   # the checker and CPU baseline only
+        from gbg_amd import workload
+        g = O.Grid3D(n)
+        m = O.Multipliers.build(g)
+        u = workload.synthetic_field(n, 2001, vector=True)
+        t0 = time.perf_counter()
+        ref, Er, Zr = O.uxw(u, g, out["config"]["kc"], m, workers=args.cpu_workers)
+        cpu_s = time.perf_counter() - t0
+        del u
+        cpu_model = platform.processor() or ""
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        cpu_model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "fields/s",
+                               "cores": args.cpu_workers, "kind": "port",
+                               "sample": f"one full {n[0]}^3 FP64 u x omega term (NumPy + scipy.fft "
+                                         f"pocketfft, workers={args.cpu_workers}), {cpu_s:.2f} s",
+                               "cpu_model": cpu_model}
+        out["rel_l2_vs_oracle"] = {"N_hat_fp64": O.rel_l2(nh_gpu, ref),
+                                   "energy": abs(E - Er) / abs(Er), "enstrophy": abs(Z - Zr) / abs(Zr)}
+        if n32 is not None:
+            out["rel_l2_vs_oracle"]["N_hat_fp32"] = O.rel_l2(n32.astype(np.complex128), ref)
+        mask = O.band_mask(m, 0.0, out["config"]["kc"])
+        out["mask_exact"] = bool(np.all(nh_gpu[:, mask == 0] == 0))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

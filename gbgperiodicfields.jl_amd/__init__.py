@@ -1,0 +1,530 @@
+"""Host-side mirror of GBGPeriodicFields.jl's spectral hot path over libgbg.
+
+The reference's public calls for this path (``src/GBGPeriodicFields.jl:281-430``)
+keep their names and argument meaning; Julia's in-place ``name!`` becomes
+``name_`` here, the allocating form keeps the plain name.  Fields live in
+device memory owned by this module (``gbg_malloc``), laid out exactly as the
+reference's column-major storage:
+
+* physical  ``(Nx, Ny, Nz[, 3[, 3]])``  -> NumPy view ``([3, [3,]] Nz, Ny, Nx)``
+* spectral  ``(Nx//2+1, Ny, Nz[, ...])`` -> NumPy view ``([...,] Nz, Ny, Nx//2+1)``
+
+Every operation runs on the GPU through the C ABI; there is no CPU path.
+Refusals raise the Python counterparts of the reference's exceptions
+(``ArgumentError``, ``DimensionMismatch``, ...).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (AliasError, ArgumentError, DimensionMismatch, GbgError,  # noqa: F401
+                   UnsupportedError, check, load)
+
+TWO_PI = 2.0 * math.pi
+
+__all__ = [
+    "Grid3D", "FieldTransformPlans", "Field", "ScalarFieldPhysical", "VectorFieldPhysical",
+    "ScalarFieldSpectral", "VectorFieldSpectral", "TensorFieldSpectral", "Stream",
+    "to_spectral_", "to_physical_", "to_spectral", "to_physical", "gradient_", "gradient",
+    "divergence_", "curl_", "curl", "inverse_curl_", "laplacian_", "inverse_laplacian_",
+    "helmholtz_projection_", "spectral_cutoff", "spectral_band", "convolve_",
+    "mean_free_projection_", "cross_", "multiply_", "inner", "l2norm", "uxw_",
+    "dealiased_product_", "dealias_cutoff", "device_count",
+]
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    st = load().gbg_device_count(C.byref(n))
+    return n.value if st == 0 else 0
+
+
+class Stream:
+    """A HIP stream owned by the library (``None`` stands for the default)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(load().gbg_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(load().gbg_stream_sync(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load().gbg_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    return None if stream is None else (stream.handle if isinstance(stream, Stream) else stream)
+
+
+@dataclass(frozen=True)
+class Grid3D:
+    """``Grid3D(grid_size, domain_size)`` (``src/Fields.jl:165-200``)."""
+
+    grid_size: tuple
+    domain_size: tuple = (TWO_PI, TWO_PI, TWO_PI)
+
+    def __post_init__(self):
+        if len(self.grid_size) != 3 or len(self.domain_size) != 3:
+            raise ArgumentError(1, "grid and domain need three axes")
+        if any(int(n) < 1 for n in self.grid_size):
+            raise ArgumentError(1, "grid sizes must be positive")
+        if any(not (float(L) > 0) or not math.isfinite(float(L)) for L in self.domain_size):
+            raise ArgumentError(1, "domain lengths must be positive and finite")
+
+    @property
+    def nh(self):
+        return self.grid_size[0] // 2 + 1
+
+    def physical_shape(self, order=0):
+        Nx, Ny, Nz = self.grid_size
+        return (3,) * order + (Nz, Ny, Nx)
+
+    def spectral_shape(self, order=0):
+        Nx, Ny, Nz = self.grid_size
+        return (3,) * order + (Nz, Ny, self.nh)
+
+
+def dealias_cutoff(n: int, L: float = TWO_PI) -> float:
+    """The 2/3-rule cutoff ``(2/3)*(N/2)*(2pi/L)`` of ``examples/HIST_CPU.jl:357``."""
+    return ((2.0 / 3.0) * (n / 2.0)) * (TWO_PI / L)
+
+
+class FieldTransformPlans:
+    """Plans for one grid and element type: ``FieldTransformPlans`` plus the
+    canonical ``FieldSpectralMultipliers`` (``src/Fields.jl:1381-1405,
+    1875-1943``), held by a ``gbg_plan``.  Rank-free: one plan serves scalar,
+    vector and tensor fields."""
+
+    def __init__(self, grid: Grid3D, dtype=np.float64):
+        dt = np.dtype(dtype)
+        if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ArgumentError(1, "element type must be Float64 or Float32")
+        self.grid = grid
+        self.dtype = dt
+        n = (C.c_int * 3)(*[int(v) for v in grid.grid_size])
+        L = (C.c_double * 3)(*[float(v) for v in grid.domain_size])
+        h = C.c_void_p()
+        check(load().gbg_plan_create(n, L, dt.itemsize, C.byref(h)))
+        self.handle = h.value
+
+    @property
+    def cdtype(self):
+        return np.complex128 if self.dtype == np.float64 else np.complex64
+
+    def device_bytes(self) -> int:
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(load().gbg_plan_query(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return c.value
+
+    def set_profiling(self, on: bool):
+        check(load().gbg_plan_set_profiling(self.handle, 1 if on else 0))
+
+    def stage_times(self):
+        buf = (C.c_double * 16)()
+        n = C.c_int(16)
+        check(load().gbg_plan_stage_times(self.handle, buf, C.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().gbg_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Field:
+    """A device-resident field: representation ('physical' | 'spectral'),
+    tensor order R (0, 1, 2), element type and grid.  ``data`` is the device
+    pointer of the contiguous column-major storage."""
+
+    def __init__(self, grid: Grid3D, representation: str, order: int, dtype=np.float64):
+        if representation not in ("physical", "spectral"):
+            raise ArgumentError(1, "representation must be physical or spectral")
+        if order not in (0, 1, 2):
+            raise DimensionMismatch(2, "tensor order must be 0, 1 or 2")
+        self.grid = grid
+        self.representation = representation
+        self.order = order
+        self.dtype = np.dtype(dtype)
+        if self.spectral:
+            self.shape = grid.spectral_shape(order)
+            self.npdtype = np.dtype(np.complex128 if self.dtype == np.float64 else np.complex64)
+        else:
+            self.shape = grid.physical_shape(order)
+            self.npdtype = self.dtype
+        self.nbytes = int(np.prod(self.shape)) * self.npdtype.itemsize
+        p = C.c_void_p()
+        check(load().gbg_malloc(C.byref(p), self.nbytes))
+        self.data = p.value
+
+    @property
+    def spectral(self):
+        return self.representation == "spectral"
+
+    @property
+    def ncomp(self):
+        return 3 ** self.order
+
+    def upload(self, arr: np.ndarray, stream=None) -> "Field":
+        a = np.ascontiguousarray(arr, dtype=self.npdtype)
+        if a.shape != self.shape:
+            raise DimensionMismatch(2, f"array shape {a.shape} != field shape {self.shape}")
+        check(load().gbg_memcpy_h2d(self.data, a.ctypes.data, self.nbytes, _sh(stream)))
+        return self
+
+    def download(self, stream=None) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.npdtype)
+        check(load().gbg_memcpy_d2h(out.ctypes.data, self.data, self.nbytes, _sh(stream)))
+        return out
+
+    def copy_from(self, other: "Field", stream=None) -> "Field":
+        _same_domain(self, other)
+        if self.shape != other.shape:
+            raise DimensionMismatch(2, "shape mismatch")
+        check(load().gbg_memcpy_d2d(self.data, other.data, self.nbytes, _sh(stream)))
+        return self
+
+    def fill_zero(self, stream=None) -> "Field":
+        check(load().gbg_memset(self.data, 0, self.nbytes, _sh(stream)))
+        return self
+
+    def similar(self, representation=None, order=None) -> "Field":
+        return Field(self.grid, representation or self.representation,
+                     self.order if order is None else order, self.dtype)
+
+    def free(self):
+        if getattr(self, "data", None):
+            load().gbg_free(self.data)
+            self.data = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def ScalarFieldPhysical(grid, dtype=np.float64):
+    return Field(grid, "physical", 0, dtype)
+
+
+def VectorFieldPhysical(grid, dtype=np.float64):
+    return Field(grid, "physical", 1, dtype)
+
+
+def ScalarFieldSpectral(grid, dtype=np.float64):
+    return Field(grid, "spectral", 0, dtype)
+
+
+def VectorFieldSpectral(grid, dtype=np.float64):
+    return Field(grid, "spectral", 1, dtype)
+
+
+def TensorFieldSpectral(grid, dtype=np.float64):
+    return Field(grid, "spectral", 2, dtype)
+
+
+# ---- argument checks mirroring the reference's refusals ------------------------
+
+def _same_domain(a: Field, b: Field):
+    """``_check_field_domain_compatibility`` (``src/Fields.jl:1225-1250``)."""
+    if a.grid != b.grid:
+        raise ArgumentError(1, "fields live on different grids")
+    if a.dtype != b.dtype:
+        raise ArgumentError(1, "field scalar types must be equal")
+
+
+def _with_plan(f: Field, plans: FieldTransformPlans):
+    """``_check_field_matches_context`` (``src/Fields.jl:1301-1305``)."""
+    if f.grid != plans.grid:
+        raise ArgumentError(1, "field grid does not match the plans' grid")
+    if f.dtype != plans.dtype:
+        raise ArgumentError(1, "field and plans scalar types must be equal")
+
+
+def _need(f: Field, rep: str, order=None):
+    if f.representation != rep:
+        raise UnsupportedError(3, f"no method for a {f.representation} field (needs {rep})")
+    if order is not None and f.order != order:
+        raise DimensionMismatch(2, f"tensor order {f.order}, expected {order}")
+
+
+# ---- transforms (src/Fields.jl:1589-1706) ---------------------------------------
+
+def to_spectral_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``to_spectral!(dst, src, plans)``: unnormalised forward rfft."""
+    _need(dst, "spectral")
+    _need(src, "physical")
+    _with_plan(dst, plans)
+    _with_plan(src, plans)
+    if dst.order != src.order:
+        raise DimensionMismatch(2, "tensor orders differ")
+    check(load().gbg_to_spectral(plans.handle, dst.data, src.data, src.ncomp, _sh(stream)))
+    return dst
+
+
+def to_physical_(dst: Field, src: Field, plans: FieldTransformPlans, scratch: Field | None = None,
+                 stream=None) -> Field:
+    """``to_physical!(dst, src, plans[, scratch])``: inverse c2r with 1/N.  The
+    3-argument form consumes ``src``; with ``scratch`` the source survives and
+    ``scratch is src`` is refused (``src/Fields.jl:1600-1619``)."""
+    _need(dst, "physical")
+    _need(src, "spectral")
+    _with_plan(dst, plans)
+    _with_plan(src, plans)
+    if dst.order != src.order:
+        raise DimensionMismatch(2, "tensor orders differ")
+    sc = None
+    if scratch is not None:
+        if scratch is src:
+            raise AliasError(4, "scratch must not be the spectral source")
+        _need(scratch, "spectral", src.order)
+        sc = scratch.data
+    check(load().gbg_to_physical(plans.handle, dst.data, src.data, src.ncomp, sc, _sh(stream)))
+    return dst
+
+
+def to_spectral(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    out = Field(src.grid, "spectral", src.order, src.dtype)
+    return to_spectral_(out, src, plans, stream)
+
+
+def to_physical(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """Allocating form: copies the source first so it is preserved."""
+    out = Field(src.grid, "physical", src.order, src.dtype)
+    tmp = src.similar()
+    return to_physical_(out, src, plans, scratch=tmp, stream=stream)
+
+
+# ---- operators (src/DifferentialOperators.jl) -------------------------------------
+
+def gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``gradient!`` scalar->vector / vector->tensor (``:812-837``)."""
+    _need(src, "spectral")
+    _need(dst, "spectral", src.order + 1)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    if src.order > 1:
+        raise DimensionMismatch(2, "gradient of a tensor field is not defined here")
+    check(load().gbg_gradient(plans.handle, dst.data, src.data, src.order, _sh(stream)))
+    return dst
+
+
+def gradient(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    return gradient_(Field(src.grid, "spectral", src.order + 1, src.dtype), src, plans, stream)
+
+
+def divergence_(dst: Field, src: Field, plans: FieldTransformPlans, kernel=None, alpha=1.0,
+                beta=0.0, stream=None) -> Field:
+    """``divergence!`` vector->scalar / tensor->vector (``:948-1037``)."""
+    _need(src, "spectral")
+    if src.order < 1:
+        raise DimensionMismatch(2, "divergence needs a vector or tensor field")
+    _need(dst, "spectral", src.order - 1)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    hb, lo, hi = _band_args(kernel)
+    check(load().gbg_divergence(plans.handle, dst.data, src.data, src.order, hb, lo, hi,
+                                float(alpha), float(beta), _sh(stream)))
+    return dst
+
+
+def curl_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``curl!`` (``:1135-1149``); dst may be src."""
+    _need(src, "spectral", 1)
+    _need(dst, "spectral", 1)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_curl(plans.handle, dst.data, src.data, _sh(stream)))
+    return dst
+
+
+def curl(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    return curl_(src.similar(), src, plans, stream)
+
+
+def inverse_curl_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``inverse_curl!`` (``:1212-1224``); dst may be src."""
+    _need(src, "spectral", 1)
+    _need(dst, "spectral", 1)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_inverse_curl(plans.handle, dst.data, src.data, _sh(stream)))
+    return dst
+
+
+def laplacian_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``laplacian!`` (``:1369-1376``)."""
+    _need(src, "spectral")
+    _need(dst, "spectral", src.order)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_laplacian(plans.handle, dst.data, src.data, src.ncomp, _sh(stream)))
+    return dst
+
+
+def inverse_laplacian_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``inverse_laplacian!`` (``:1457-1464``)."""
+    _need(src, "spectral")
+    _need(dst, "spectral", src.order)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_inverse_laplacian(plans.handle, dst.data, src.data, src.ncomp, _sh(stream)))
+    return dst
+
+
+def helmholtz_projection_(dst: Field, src: Field, plans: FieldTransformPlans, kernel=None,
+                          alpha=1.0, beta=0.0, stream=None) -> Field:
+    """``helmholtz_projection!(dst, src, m[, kernel, alpha, beta])`` (``:1574-1593``)."""
+    _need(src, "spectral", 1)
+    _need(dst, "spectral", 1)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    hb, lo, hi = _band_args(kernel)
+    check(load().gbg_helmholtz(plans.handle, dst.data, src.data, hb, lo, hi, float(alpha),
+                               float(beta), _sh(stream)))
+    return dst
+
+
+@dataclass(frozen=True)
+class RadialBand:
+    """``spectral_band``/``spectral_cutoff`` as a radial symbol (``src/Convolution.jl:1832-1882``)."""
+
+    k_lo: float
+    k_hi: float
+
+
+def spectral_band(plans: FieldTransformPlans, k_lo: float, k_hi: float) -> RadialBand:
+    if not (math.isfinite(k_lo) and math.isfinite(k_hi)):
+        raise ArgumentError(1, f"band edges must be finite, got ({k_lo}, {k_hi})")
+    if not (0 <= k_lo <= k_hi):
+        raise ArgumentError(1, f"band edges must satisfy 0 <= low <= high, got ({k_lo}, {k_hi})")
+    return RadialBand(float(k_lo), float(k_hi))
+
+
+def spectral_cutoff(plans: FieldTransformPlans, kc: float) -> RadialBand:
+    return spectral_band(plans, 0.0, kc)
+
+
+def _band_args(kernel):
+    if kernel is None:
+        return 0, 0.0, 0.0
+    if not isinstance(kernel, RadialBand):
+        raise UnsupportedError(3, "only radial band kernels are supported")
+    return 1, kernel.k_lo, kernel.k_hi
+
+
+def convolve_(dst: Field, src: Field, kernel: RadialBand, plans: FieldTransformPlans, alpha=1.0,
+              beta=0.0, stream=None) -> Field:
+    """``convolve!(dst, src, kernel[, alpha, beta])`` for a radial band kernel
+    (``src/Convolution.jl:1035-1065``)."""
+    _need(src, "spectral")
+    _need(dst, "spectral", src.order)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    _, lo, hi = _band_args(kernel)
+    check(load().gbg_convolve_band(plans.handle, dst.data, src.data, src.ncomp, lo, hi,
+                                   float(alpha), float(beta), _sh(stream)))
+    return dst
+
+
+def mean_free_projection_(f: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``mean_free_projection!`` (``src/Fields.jl:3124-3128``)."""
+    _need(f, "spectral")
+    _with_plan(f, plans)
+    check(load().gbg_mean_free(plans.handle, f.data, f.ncomp, _sh(stream)))
+    return f
+
+
+# ---- physical products (src/FieldAlgebra.jl) ---------------------------------------
+
+def cross_(dst: Field, u: Field, v: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``cross!(dst, u, v)`` (``src/FieldAlgebra.jl:1088-1092``)."""
+    for f in (dst, u, v):
+        _need(f, "physical", 1)
+        _with_plan(f, plans)
+    check(load().gbg_cross(plans.handle, dst.data, u.data, v.data, _sh(stream)))
+    return dst
+
+
+def multiply_(dst: Field, a: Field, b: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``multiply!(dst, a, b)`` for scalar fields (``src/FieldAlgebra.jl:778``)."""
+    for f in (dst, a, b):
+        _need(f, "physical", 0)
+        _with_plan(f, plans)
+    check(load().gbg_multiply(plans.handle, dst.data, a.data, b.data, _sh(stream)))
+    return dst
+
+
+# ---- reductions (src/Reductions.jl) -------------------------------------------------
+
+def inner(f: Field, g: Field, plans: FieldTransformPlans, stream=None) -> float:
+    """Spectral ``inner(f, g)`` through Parseval (``src/Reductions.jl:883-925``)."""
+    _need(f, "spectral")
+    _need(g, "spectral")
+    _with_plan(f, plans)
+    _same_domain(f, g)
+    if f.order != g.order:
+        raise DimensionMismatch(2, f"sections have different tensor orders: {f.order} and {g.order}")
+    out = C.c_double(0.0)
+    check(load().gbg_inner_spectral(plans.handle, f.data, g.data, f.ncomp, C.byref(out), _sh(stream)))
+    return out.value
+
+
+def l2norm(f: Field, plans: FieldTransformPlans, stream=None) -> float:
+    """``l2norm(f) = sqrt(inner(f, f))`` (``src/Reductions.jl:947-948``)."""
+    return math.sqrt(inner(f, f, plans, stream))
+
+
+# ---- fused pipelines ---------------------------------------------------------------
+
+def uxw_(n_hat: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | None = None,
+         fused: bool = True, reductions: bool = True, stream=None):
+    """The dealiased rotational nonlinear term ``N^ = Q M P F(u x w)`` of
+    ``examples/HIST_CPU.jl:229-247`` in one call.  Returns ``(E, Z)`` (FP64)
+    when ``reductions`` is true, else ``None`` (no synchronisation)."""
+    _need(u_hat, "spectral", 1)
+    _need(n_hat, "spectral", 1)
+    _with_plan(u_hat, plans)
+    _same_domain(n_hat, u_hat)
+    if kc is None:
+        kc = dealias_cutoff(plans.grid.grid_size[0], plans.grid.domain_size[0])
+    flags = 0 if fused else _lib.GBG_UXW_UNFUSED
+    if reductions:
+        E, Z = C.c_double(0.0), C.c_double(0.0)
+        check(load().gbg_uxw(plans.handle, u_hat.data, n_hat.data, float(kc), C.byref(E), C.byref(Z),
+                             flags, _sh(stream)))
+        return E.value, Z.value
+    check(load().gbg_uxw(plans.handle, u_hat.data, n_hat.data, float(kc), None, None, flags,
+                         _sh(stream)))
+    return None
+
+
+def dealiased_product_(p_hat: Field, a_hat: Field, b_hat: Field, plans: FieldTransformPlans,
+                       kc: float | None = None, stream=None) -> Field:
+    """``p^ = M F(F^-1(M a^) F^-1(M b^))`` for scalar fields."""
+    for f in (p_hat, a_hat, b_hat):
+        _need(f, "spectral", 0)
+        _with_plan(f, plans)
+    if kc is None:
+        kc = dealias_cutoff(plans.grid.grid_size[0], plans.grid.domain_size[0])
+    check(load().gbg_dealiased_product(plans.handle, p_hat.data, a_hat.data, b_hat.data, float(kc),
+                                       _sh(stream)))
+    return p_hat

@@ -1,0 +1,117 @@
+"""ctypes binding of libgbg.so (declared in include/gbg.h).
+
+The library is loaded from this directory; a missing library raises at once
+-- there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgbg.so")
+
+GBG_OK = 0
+STATUS_NAMES = {0: "GBG_OK", 1: "GBG_E_ARG", 2: "GBG_E_DIM", 3: "GBG_E_UNSUPPORTED",
+                4: "GBG_E_ALIAS", 5: "GBG_E_OOM", 6: "GBG_E_DEVICE"}
+
+GBG_UXW_UNFUSED = 1
+
+vp = C.c_void_p
+st = C.c_int
+sz = C.c_size_t
+dbl = C.c_double
+i32 = C.c_int
+dp = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); the exact set of symbols include/gbg.h declares
+SIGNATURES = {
+    "gbg_last_error": (C.c_char_p, []),
+    "gbg_version": (C.c_char_p, []),
+    "gbg_set_device": (st, [i32]),
+    "gbg_device_count": (st, [C.POINTER(C.c_int)]),
+    "gbg_malloc": (st, [C.POINTER(vp), sz]),
+    "gbg_free": (st, [vp]),
+    "gbg_memcpy_h2d": (st, [vp, vp, sz, vp]),
+    "gbg_memcpy_d2h": (st, [vp, vp, sz, vp]),
+    "gbg_memcpy_d2d": (st, [vp, vp, sz, vp]),
+    "gbg_memset": (st, [vp, i32, sz, vp]),
+    "gbg_stream_create": (st, [C.POINTER(vp)]),
+    "gbg_stream_destroy": (st, [vp]),
+    "gbg_stream_sync": (st, [vp]),
+    "gbg_plan_create": (st, [C.POINTER(C.c_int), C.POINTER(C.c_double), i32, C.POINTER(vp)]),
+    "gbg_plan_destroy": (st, [vp]),
+    "gbg_plan_query": (st, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+    "gbg_to_spectral": (st, [vp, vp, vp, i32, vp]),
+    "gbg_to_physical": (st, [vp, vp, vp, i32, vp, vp]),
+    "gbg_gradient": (st, [vp, vp, vp, i32, vp]),
+    "gbg_divergence": (st, [vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl, vp]),
+    "gbg_curl": (st, [vp, vp, vp, vp]),
+    "gbg_inverse_curl": (st, [vp, vp, vp, vp]),
+    "gbg_laplacian": (st, [vp, vp, vp, i32, vp]),
+    "gbg_inverse_laplacian": (st, [vp, vp, vp, i32, vp]),
+    "gbg_helmholtz": (st, [vp, vp, vp, i32, dbl, dbl, dbl, dbl, vp]),
+    "gbg_convolve_band": (st, [vp, vp, vp, i32, dbl, dbl, dbl, dbl, vp]),
+    "gbg_mean_free": (st, [vp, vp, i32, vp]),
+    "gbg_cross": (st, [vp, vp, vp, vp, vp]),
+    "gbg_multiply": (st, [vp, vp, vp, vp, vp]),
+    "gbg_inner_spectral": (st, [vp, vp, vp, i32, dp, vp]),
+    "gbg_uxw": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
+    "gbg_dealiased_product": (st, [vp, vp, vp, vp, dbl, vp]),
+    "gbg_plan_set_profiling": (st, [vp, i32]),
+    "gbg_plan_stage_times": (st, [vp, dp, C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libgbg.so once and attach the typed signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libgbg.so not built at {p}: run __graft_entry__.build() "
+                           "or python gbgperiodicfields.jl_amd/build_lib.py")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GbgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ArgumentError(GbgError, ValueError):
+    pass
+
+
+class DimensionMismatch(GbgError, ValueError):
+    pass
+
+
+class UnsupportedError(GbgError, NotImplementedError):
+    pass
+
+
+class AliasError(GbgError, ValueError):
+    pass
+
+
+_EXC = {1: ArgumentError, 2: DimensionMismatch, 3: UnsupportedError, 4: AliasError}
+
+
+def check(status: int):
+    """Raise the Python counterpart of the reference's refusal for a status."""
+    if status == GBG_OK:
+        return
+    msg = load().gbg_last_error().decode(errors="replace")
+    raise _EXC.get(status, GbgError)(status, msg)

@@ -1,0 +1,200 @@
+// Pencil FFT passes: the 3-D real-to-complex transform of periodic fields.
+//
+// forward (to_spectral!):  x-pass R2C (phys -> spec), y-pass C2C, z-pass C2C (in place)
+// inverse (to_physical!):  z-pass C2C, y-pass C2C, x-pass C2R (1/N applied here)
+//
+// Strided passes load B consecutive pencils of the fast index into LDS, so
+// every global row segment is B contiguous complex elements; the x passes
+// pair two real rows into one complex transform (z = a + i b).
+#include "fft_core.hpp"
+
+namespace gbg {
+
+// Strided pencil addressing: pencil p, element e along the axis, component c
+//   offset = c*cstride + (p / inner)*ostride + (p % inner) + e*se
+struct Strided {
+  long long npencil, inner, ostride, se, cstride;
+  int B, bshift, ld;
+};
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(kNT) void k_c2c_strided(const cpx<T>* in, cpx<T>* out, Strided d,
+                                                     AxisPlan ap, const cpx<T>* __restrict__ tw,
+                                                     T scale) {
+  __shared__ cpx<T> s[2 * Lds<T>::kElems];
+  const long long coff = (long long)blockIdx.y * d.cstride;
+  const int pl = threadIdx.x & (d.B - 1);
+  const int e0 = threadIdx.x >> d.bshift;
+  const int estep = kNT >> d.bshift;
+  const long long p = (long long)blockIdx.x * d.B + pl;
+  const bool valid = p < d.npencil;
+  const long long base = coff + (valid ? (p / d.inner) * d.ostride + (p % d.inner) : 0);
+  const int n = ap.n;
+  for (int e = e0; e < n; e += estep) {
+    cpx<T> v = mk<T>(0, 0);
+    if (valid) v = in[base + (long long)e * d.se];
+    s[pl * d.ld + e] = v;
+  }
+  __syncthreads();
+  const cpx<T>* r = fft_lds<T, DIR>(s, s + Lds<T>::kElems, d.ld, ap, d.B, tw);
+  if (valid) {
+    for (int e = e0; e < n; e += estep) out[base + (long long)e * d.se] = cscale(r[pl * d.ld + e], scale);
+  }
+}
+
+// x-pass R2C: B row pairs per workgroup.  Row r of component data is at
+// r*nx (physical) / r*nh (spectral); rows of all components are contiguous.
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_r2c_x(const T* __restrict__ in, cpx<T>* __restrict__ out,
+                                               long long nrows, int nx, int nh, int B, int ld,
+                                               AxisPlan ap, const cpx<T>* __restrict__ tw) {
+  __shared__ cpx<T> s[2 * Lds<T>::kElems];
+  const long long pair0 = (long long)blockIdx.x * B;
+  for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
+    const int q = idx / nx, x = idx - q * nx;
+    const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
+    const T a = r0 < nrows ? in[r0 * nx + x] : T(0);
+    const T b = r1 < nrows ? in[r1 * nx + x] : T(0);
+    s[q * ld + x] = mk<T>(a, b);
+  }
+  __syncthreads();
+  const cpx<T>* rs = fft_lds<T, -1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  for (int idx = threadIdx.x; idx < B * nh; idx += kNT) {
+    const int q = idx / nh, k = idx - q * nh;
+    const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
+    if (r0 >= nrows) continue;
+    const cpx<T> z = rs[q * ld + k];
+    const cpx<T> zc = rs[q * ld + (k == 0 ? 0 : nx - k)];
+    // A = (Z_k + conj Z_{n-k}) / 2,  B = (Z_k - conj Z_{n-k}) / (2i)
+    out[r0 * nh + k] = mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y));
+    if (r1 < nrows) out[r1 * nh + k] = mk<T>(T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x));
+  }
+}
+
+// x-pass C2R: Z_k = A_k + i B_k over the Hermitian completion, with the
+// self-conjugate entries (k = 0 and the even-axis Nyquist) taken real, as a
+// c2r transform treats them.  Output scaled by `scale` (1/N of the 3-D
+// inverse).
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_c2r_x(const cpx<T>* __restrict__ in, T* __restrict__ out,
+                                               long long nrows, int nx, int nh, int B, int ld,
+                                               AxisPlan ap, const cpx<T>* __restrict__ tw, T scale) {
+  __shared__ cpx<T> s[2 * Lds<T>::kElems];
+  const long long pair0 = (long long)blockIdx.x * B;
+  const int nyq = (nx % 2 == 0) ? nx / 2 : -1;
+  for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
+    const int q = idx / nx, k = idx - q * nx;
+    const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
+    const bool upper = k >= nh;
+    const int kk = upper ? nx - k : k;
+    cpx<T> a = r0 < nrows ? in[r0 * nh + kk] : mk<T>(0, 0);
+    cpx<T> b = r1 < nrows ? in[r1 * nh + kk] : mk<T>(0, 0);
+    if (kk == 0 || kk == nyq) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    if (upper) {
+      a.y = -a.y;
+      b.y = -b.y;
+    }
+    s[q * ld + k] = mk<T>(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  const cpx<T>* rs = fft_lds<T, 1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
+    const int q = idx / nx, x = idx - q * nx;
+    const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
+    if (r0 >= nrows) continue;
+    const cpx<T> z = rs[q * ld + x];
+    out[r0 * nx + x] = z.x * scale;
+    if (r1 < nrows) out[r1 * nx + x] = z.y * scale;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------
+static int ilog2(int b) {
+  int s = 0;
+  while ((1 << s) < b) ++s;
+  return s;
+}
+
+template <typename T>
+static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* in, cpx<T>* out,
+                                 int ncomp, T scale, hipStream_t s) {
+  const AxisPlan& ap = p->ax[axis];
+  if (ap.n == 1 && in == out) return GBG_OK;
+  Strided d;
+  const long long nh = p->nh, ny = p->n[1];
+  if (axis == 1) {        // y: pencils over (z, kx)
+    d.npencil = (long long)p->n[2] * nh;
+    d.inner = nh;
+    d.ostride = nh * ny;
+    d.se = nh;
+  } else {                // z: pencils over (ky, kx)
+    d.npencil = ny * nh;
+    d.inner = ny * nh;
+    d.ostride = 0;
+    d.se = ny * nh;
+  }
+  d.cstride = (long long)p->M;
+  d.B = pick_b(ap.n, Lds<T>::kCap);
+  d.bshift = ilog2(d.B);
+  d.ld = ap.n + 1;
+  dim3 grid((unsigned)((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(p->tw[axis]);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_c2c_strided<T, -1>), grid, dim3(kNT), 0, s, in, out, d, ap, tw, scale);
+  else
+    hipLaunchKernelGGL((k_c2c_strided<T, 1>), grid, dim3(kNT), 0, s, in, out, d, ap, tw, scale);
+  return check_launch("c2c pass");
+}
+
+template <typename T>
+static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, hipStream_t s) {
+  const int nx = p->n[0], nh = p->nh;
+  const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
+  const int B = pick_b(nx, Lds<T>::kCap);
+  const long long pairs = (nrows + 1) / 2;
+  dim3 grid((unsigned)((pairs + B - 1) / B));
+  hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
+                     p->ax[0], reinterpret_cast<const cpx<T>*>(p->tw[0]));
+  gbg_status st = check_launch("r2c x pass");
+  if (st) return st;
+  if ((st = launch_strided<T>(p, 1, -1, spec, spec, ncomp, T(1), s))) return st;
+  return launch_strided<T>(p, 2, -1, spec, spec, ncomp, T(1), s);
+}
+
+template <typename T>
+static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, int ncomp,
+                           hipStream_t s) {
+  gbg_status st;
+  if ((st = launch_strided<T>(p, 2, 1, spec_in, work, ncomp, T(1), s))) return st;
+  if ((st = launch_strided<T>(p, 1, 1, work, work, ncomp, T(1), s))) return st;
+  const int nx = p->n[0], nh = p->nh;
+  const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
+  const int B = pick_b(nx, Lds<T>::kCap);
+  const long long pairs = (nrows + 1) / 2;
+  dim3 grid((unsigned)((pairs + B - 1) / B));
+  const T scale = T(1.0 / ((double)p->n[0] * (double)p->n[1] * (double)p->n[2]));
+  hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,
+                     p->ax[0], reinterpret_cast<const cpx<T>*>(p->tw[0]), scale);
+  return check_launch("c2r x pass");
+}
+
+gbg_status fwd_transform(gbg_plan* p, void* spec, const void* phys, int ncomp, hipStream_t s) {
+  if (p->elem == 8)
+    return fwd_impl<double>(p, (cpx<double>*)spec, (const double*)phys, ncomp, s);
+  return fwd_impl<float>(p, (cpx<float>*)spec, (const float*)phys, ncomp, s);
+}
+
+gbg_status inv_transform(gbg_plan* p, void* phys, void* spec_in, void* spec_work, int ncomp,
+                         hipStream_t s) {
+  if (p->elem == 8)
+    return inv_impl<double>(p, (double*)phys, (cpx<double>*)spec_in, (cpx<double>*)spec_work,
+                            ncomp, s);
+  return inv_impl<float>(p, (float*)phys, (cpx<float>*)spec_in, (cpx<float>*)spec_work, ncomp, s);
+}
+
+}  // namespace gbg

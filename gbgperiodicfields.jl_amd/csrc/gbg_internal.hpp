@@ -1,0 +1,148 @@
+// Internal declarations shared by the libgbg translation units: the complex
+// type, the plan object, status plumbing and the kernel-launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gbg.h"
+
+namespace gbg {
+
+// ---------------------------------------------------------------------------
+// complex arithmetic (interleaved re/im, the layout of Julia's Complex{T})
+// ---------------------------------------------------------------------------
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+  T x, y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> mk(T x, T y) {
+  cpx<T> r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+  return mk<T>(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
+  return mk<T>(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) {
+  return mk<T>(a.x * s, a.y * s);
+}
+// i * a
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> cmuli(cpx<T> a) {
+  return mk<T>(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// 1-D transform plan: a mixed-radix factorisation of the axis length
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 16;
+struct AxisPlan {
+  int n;
+  int nst;
+  int radix[kMaxStages];
+};
+
+// Factor n into the radices the butterflies implement (8, 4, 2, 3, 5, 7, 11,
+// 13).  Returns false when n has a larger prime factor.
+bool factor_axis(int n, AxisPlan* ap);
+
+// ---------------------------------------------------------------------------
+// The plan object behind gbg_plan*
+// ---------------------------------------------------------------------------
+constexpr int kMaxEvents = 16;
+constexpr int kRedBlocks = 1024;   // fixed reduction grid: deterministic sums
+
+}  // namespace gbg
+
+struct gbg_plan {
+  int n[3];          // Nx, Ny, Nz
+  double L[3];       // box lengths
+  int elem;          // 4 or 8
+  int nh;            // Nx/2 + 1
+  size_t M;          // stored modes per component
+  size_t P;          // grid points per component
+  int device;
+  gbg::AxisPlan ax[3];
+  // FP64 per-axis tables (device): derivative family and squared family
+  double* kd[3];
+  double* ks[3];
+  // twiddle tables, n entries e^{-2 pi i m / n} in the plan's precision
+  void* tw[3];
+  // pipeline work area (allocated on first use, kept)
+  void* work;
+  size_t work_bytes;
+  // reduction partials and results (FP64)
+  double* red_part;
+  double* red_out;
+  size_t device_bytes;
+  // profiling events
+  int profiling;
+  int nev;
+  hipEvent_t ev[gbg::kMaxEvents];
+  hipStream_t ev_stream;
+};
+
+namespace gbg {
+
+// status plumbing --------------------------------------------------------------
+void set_error(const std::string& msg);
+gbg_status fail(gbg_status st, const std::string& msg);
+gbg_status check_hip(hipError_t e, const char* what);
+gbg_status check_launch(const char* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ensure the plan's work area holds at least `bytes`
+gbg_status ensure_work(gbg_plan* p, size_t bytes);
+
+// event marks for the profiling aid
+void prof_begin(gbg_plan* p, hipStream_t s);
+void prof_mark(gbg_plan* p, hipStream_t s);
+
+// ---- launchers implemented in the .hip translation units ---------------------
+// transforms (gbg_fft.hip)
+gbg_status fwd_transform(gbg_plan* p, void* spec, const void* phys, int ncomp, hipStream_t s);
+gbg_status inv_transform(gbg_plan* p, void* phys, void* spec_in, void* spec_work, int ncomp,
+                         hipStream_t s);
+
+// operators (gbg_ops.hip)
+gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, hipStream_t s);
+gbg_status op_divergence(gbg_plan* p, void* dst, const void* src, int src_order, int has_band,
+                         double lo, double hi, double alpha, double beta, hipStream_t s);
+gbg_status op_curl(gbg_plan* p, void* dst, const void* src, int inverse, hipStream_t s);
+gbg_status op_laplacian(gbg_plan* p, void* dst, const void* src, int ncomp, int inverse,
+                        hipStream_t s);
+gbg_status op_helmholtz(gbg_plan* p, void* dst, const void* src, int has_band, double lo,
+                        double hi, double alpha, double beta, hipStream_t s);
+gbg_status op_convolve_band(gbg_plan* p, void* dst, const void* src, int ncomp, double lo,
+                            double hi, double alpha, double beta, hipStream_t s);
+gbg_status op_mean_free(gbg_plan* p, void* spec, int ncomp, hipStream_t s);
+gbg_status op_cross(gbg_plan* p, void* dst, const void* u, const void* v, hipStream_t s);
+gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hipStream_t s);
+// Parseval inner into p->red_out[slot] (device); no synchronisation
+gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int slot, hipStream_t s);
+
+// pipelines (gbg_uxw.hip)
+gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
+                    hipStream_t s);
+gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
+                                  double kc, hipStream_t s);
+
+}  // namespace gbg

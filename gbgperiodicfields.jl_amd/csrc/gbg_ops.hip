@@ -1,0 +1,417 @@
+// Spectral operators, physical products and the Parseval reduction as
+// standalone streaming kernels, one work item per mode (or grid point).
+// Per-mode arithmetic follows the reference kernels expression for
+// expression (src/DifferentialOperators.jl, src/Convolution.jl,
+// src/FieldAlgebra.jl, src/Reductions.jl); wavenumbers come from the plan's
+// FP64 tables and are rounded to T where the field is Float32, while the
+// band mask is always evaluated in FP64.
+#include "gbg_internal.hpp"
+
+namespace gbg {
+
+constexpr int kOpNT = 256;
+
+// decode a linear mode index into (kx, ky, kz)
+struct ModeIdx {
+  unsigned nh, ny;
+  __device__ __forceinline__ void operator()(unsigned idx, unsigned& kx, unsigned& ky,
+                                             unsigned& kz) const {
+    kx = idx % nh;
+    const unsigned t = idx / nh;
+    ky = t % ny;
+    kz = t / ny;
+  }
+};
+
+struct Tables {
+  const double* kd1;
+  const double* kd2;
+  const double* kd3;
+  const double* ks1;
+  const double* ks2;
+  const double* ks3;
+};
+
+static Tables tables_of(gbg_plan* p) {
+  return Tables{p->kd[0], p->kd[1], p->kd[2], p->ks[0], p->ks[1], p->ks[2]};
+}
+
+// spectral_band membership in FP64: lo2 <= (k1^2 + k2^2) + k3^2 <= hi2
+__device__ __forceinline__ bool in_band(const Tables& t, unsigned kx, unsigned ky, unsigned kz,
+                                        double lo2, double hi2) {
+  const double q = (t.ks1[kx] + t.ks2[ky]) + t.ks3[kz];
+  return lo2 <= q && q <= hi2;
+}
+
+static unsigned nblocks(size_t n) { return (unsigned)((n + kOpNT - 1) / kOpNT); }
+
+// ---- gradient ----------------------------------------------------------------
+// scalar -> vector: d_j = (i a_j) s      (_scalar_gradient_kernel!, :357-368)
+// vector -> tensor: d_ij = (i a_j) u_i   (_vector_gradient_kernel!, :414-441)
+//   tensor component (i, j) is stored at block i + 3 j (Julia column-major).
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_gradient(cpx<T>* __restrict__ dst,
+                                                    const cpx<T>* __restrict__ src, unsigned M,
+                                                    int nsrc, ModeIdx mi, Tables t) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T a[3] = {T(t.kd1[kx]), T(t.kd2[ky]), T(t.kd3[kz])};
+  for (int i = 0; i < nsrc; ++i) {
+    const cpx<T> v = src[(size_t)i * M + idx];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      // (i a) * v with the factor formed first: (0 + i a)(x + i y)
+      const cpx<T> f = mk<T>(T(0), a[j]);
+      dst[(size_t)(i + nsrc * j) * M + idx] = cmul(f, v);
+    }
+  }
+}
+
+// ---- divergence --------------------------------------------------------------
+// i ((a1 s1 + a2 s2) + a3 s3), optionally alpha * g * (.) + beta * dst
+//   (_apply_spectral_divergence!, :120-139)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_divergence(cpx<T>* dst, const cpx<T>* src, unsigned M,
+                                                      int nout, ModeIdx mi, Tables t, int has_band,
+                                                      double lo2, double hi2, T alpha, T beta) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T a1 = T(t.kd1[kx]), a2 = T(t.kd2[ky]), a3 = T(t.kd3[kz]);
+  const T g = has_band ? (in_band(t, kx, ky, kz, lo2, hi2) ? T(1) : T(0)) : T(1);
+  for (int i = 0; i < nout; ++i) {
+    // source block (i, j) of a tensor is i + nout*j; a vector has nout = 1
+    const cpx<T> s1 = src[(size_t)(i + nout * 0) * M + idx];
+    const cpx<T> s2 = src[(size_t)(i + nout * 1) * M + idx];
+    const cpx<T> s3 = src[(size_t)(i + nout * 2) * M + idx];
+    const cpx<T> sum = cadd(cadd(cscale(s1, a1), cscale(s2, a2)), cscale(s3, a3));
+    cpx<T> r = cmul(mk<T>(T(0), T(1)), sum);
+    if (has_band || alpha != T(1) || beta != T(0)) {
+      r = cscale(r, alpha * g);
+      if (beta != T(0)) r = cadd(r, cscale(dst[(size_t)i * M + idx], beta));
+    }
+    dst[(size_t)i * M + idx] = r;
+  }
+}
+
+// ---- curl / inverse curl -----------------------------------------------------
+// i (a2 u3 - a3 u2, a3 u1 - a1 u3, a1 u2 - a2 u1)          (_curl_kernel!, :502-523)
+// the same times (|a|^2 == 0 ? 0 : 1/|a|^2)                (_inverse_curl_kernel!, :580-605)
+// All reads of a mode precede its writes: dst may alias src.
+template <typename T, bool INV>
+__global__ __launch_bounds__(kOpNT) void k_curl(cpx<T>* dst, const cpx<T>* src, unsigned M,
+                                                ModeIdx mi, Tables t) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T a1 = T(t.kd1[kx]), a2 = T(t.kd2[ky]), a3 = T(t.kd3[kz]);
+  const cpx<T> u1 = src[idx], u2 = src[(size_t)M + idx], u3 = src[2 * (size_t)M + idx];
+  cpx<T> c1 = cmuli(csub(cscale(u3, a2), cscale(u2, a3)));
+  cpx<T> c2 = cmuli(csub(cscale(u1, a3), cscale(u3, a1)));
+  cpx<T> c3 = cmuli(csub(cscale(u2, a1), cscale(u1, a2)));
+  if (INV) {
+    const T sq = a1 * a1 + a2 * a2 + a3 * a3;
+    const T rec = sq == T(0) ? T(0) : T(1) / sq;
+    c1 = cscale(c1, rec);
+    c2 = cscale(c2, rec);
+    c3 = cscale(c3, rec);
+  }
+  dst[idx] = c1;
+  dst[(size_t)M + idx] = c2;
+  dst[2 * (size_t)M + idx] = c3;
+}
+
+// ---- Laplacian / inverse Laplacian --------------------------------------------
+// -|k|^2 f   and   (-f) * (|k|^2 == 0 ? 0 : inv(|k|^2))   (:147-164)
+// |k|^2 = (k1^2 + k2^2) + k3^2 from the squared family, in T.
+template <typename T, bool INV>
+__global__ __launch_bounds__(kOpNT) void k_laplacian(cpx<T>* dst, const cpx<T>* src, unsigned M,
+                                                     int ncomp, ModeIdx mi, Tables t) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T q = (T(t.ks1[kx]) + T(t.ks2[ky])) + T(t.ks3[kz]);
+  const T f = INV ? (q == T(0) ? T(0) : T(1) / q) : -q;
+  for (int c = 0; c < ncomp; ++c) {
+    const cpx<T> v = src[(size_t)c * M + idx];
+    dst[(size_t)c * M + idx] = INV ? cscale(mk<T>(-v.x, -v.y), f) : cscale(v, f);
+  }
+}
+
+// ---- Helmholtz (Leray) projection ----------------------------------------------
+// L = (a.u) / (a.a == 0 ? 1 : a.a);  u_i <- alpha g (u_i - a_i L) (+ beta d_i)
+//   (_helmholtz_projection_kernel!, :191-230)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_helmholtz(cpx<T>* dst, const cpx<T>* src, unsigned M,
+                                                     ModeIdx mi, Tables t, int has_band, double lo2,
+                                                     double hi2, T alpha, T beta) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T a1 = T(t.kd1[kx]), a2 = T(t.kd2[ky]), a3 = T(t.kd3[kz]);
+  const T sq = a1 * a1 + a2 * a2 + a3 * a3;
+  const cpx<T> u1 = src[idx], u2 = src[(size_t)M + idx], u3 = src[2 * (size_t)M + idx];
+  const cpx<T> dot = cadd(cadd(cscale(u1, a1), cscale(u2, a2)), cscale(u3, a3));
+  const T den = sq == T(0) ? T(1) : sq;
+  const cpx<T> lon = mk<T>(dot.x / den, dot.y / den);
+  const T g = has_band ? (in_band(t, kx, ky, kz, lo2, hi2) ? T(1) : T(0)) : T(1);
+  const T sc = alpha * g;
+  cpx<T> p1 = cscale(csub(u1, cscale(lon, a1)), sc);
+  cpx<T> p2 = cscale(csub(u2, cscale(lon, a2)), sc);
+  cpx<T> p3 = cscale(csub(u3, cscale(lon, a3)), sc);
+  if (beta != T(0)) {
+    p1 = cadd(p1, cscale(dst[idx], beta));
+    p2 = cadd(p2, cscale(dst[(size_t)M + idx], beta));
+    p3 = cadd(p3, cscale(dst[2 * (size_t)M + idx], beta));
+  }
+  dst[idx] = p1;
+  dst[(size_t)M + idx] = p2;
+  dst[2 * (size_t)M + idx] = p3;
+}
+
+// ---- radial band convolution (the 2/3 mask) ---------------------------------------
+// alpha * sigma(|k|^2) * f (+ beta dst)   (_radial_convolution_pass!, src/Convolution.jl:756-787)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_band(cpx<T>* dst, const cpx<T>* src, unsigned M,
+                                                int ncomp, ModeIdx mi, Tables t, double lo2,
+                                                double hi2, T alpha, T beta) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M) return;
+  unsigned kx, ky, kz;
+  mi(idx, kx, ky, kz);
+  const T g = in_band(t, kx, ky, kz, lo2, hi2) ? T(1) : T(0);
+  const T f = alpha * g;
+  for (int c = 0; c < ncomp; ++c) {
+    cpx<T> r = cscale(src[(size_t)c * M + idx], f);
+    if (beta != T(0)) r = cadd(r, cscale(dst[(size_t)c * M + idx], beta));
+    dst[(size_t)c * M + idx] = r;
+  }
+}
+
+// ---- mean-free projection: zero the k = 0 coefficient of each component -------------
+template <typename T>
+__global__ void k_mean_free(cpx<T>* spec, size_t M, int ncomp) {
+  const int c = threadIdx.x;
+  if (c < ncomp) spec[(size_t)c * M] = mk<T>(T(0), T(0));
+}
+
+// ---- physical products --------------------------------------------------------------
+// (u x v)_i = u_j v_k - u_k v_j   (_cross_formula, src/FiberSections.jl:641-652)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_cross(T* dst, const T* u, const T* v, size_t P) {
+  const size_t i = (size_t)blockIdx.x * kOpNT + threadIdx.x;
+  if (i >= P) return;
+  const T u1 = u[i], u2 = u[P + i], u3 = u[2 * P + i];
+  const T v1 = v[i], v2 = v[P + i], v3 = v[2 * P + i];
+  dst[i] = u2 * v3 - u3 * v2;
+  dst[P + i] = u3 * v1 - u1 * v3;
+  dst[2 * P + i] = u1 * v2 - u2 * v1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_multiply(T* dst, const T* a, const T* b, size_t P) {
+  const size_t i = (size_t)blockIdx.x * kOpNT + threadIdx.x;
+  if (i >= P) return;
+  dst[i] = a[i] * b[i];
+}
+
+// ---- Parseval inner, FP64 accumulation in a fixed order --------------------------------
+// sum_c sum_modes w(kx) Re(f conj g), w = 1 on kx = 0 and the even Nyquist plane,
+// else 2; result / N^2   (src/Reductions.jl:883-925, weight :956-965)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_inner_part(const cpx<T>* f, const cpx<T>* g, size_t M,
+                                                      int ncomp, unsigned nh, int nyq,
+                                                      double* part) {
+  __shared__ double red[kOpNT];
+  const size_t total = M * (size_t)ncomp;
+  const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
+  const size_t lo = chunk * blockIdx.x;
+  const size_t hi = lo + chunk < total ? lo + chunk : total;
+  double acc = 0.0;
+  for (size_t i = lo + threadIdx.x; i < hi; i += kOpNT) {
+    const size_t m = i % M;
+    const unsigned kx = (unsigned)(m % nh);
+    const double w = (kx == 0 || (int)kx == nyq) ? 1.0 : 2.0;
+    const cpx<T> a = f[i], b = g[i];
+    acc += w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kOpNT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kOpNT) void k_inner_final(const double* part, int nparts, double norm,
+                                                       double* out) {
+  __shared__ double red[kOpNT];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kOpNT) acc += part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kOpNT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0] / norm;
+}
+
+// ----------------------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------------------
+static ModeIdx modes_of(gbg_plan* p) { return ModeIdx{(unsigned)p->nh, (unsigned)p->n[1]}; }
+
+gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  const int nsrc = src_order == 0 ? 1 : 3;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_gradient<double>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                       (const cpx<double>*)src, M, nsrc, modes_of(p), tables_of(p));
+  else
+    hipLaunchKernelGGL((k_gradient<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                       (const cpx<float>*)src, M, nsrc, modes_of(p), tables_of(p));
+  return check_launch("gradient");
+}
+
+gbg_status op_divergence(gbg_plan* p, void* dst, const void* src, int src_order, int has_band,
+                         double lo, double hi, double alpha, double beta, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  const int nout = src_order == 1 ? 1 : 3;
+  const double lo2 = lo * lo, hi2 = hi * hi;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_divergence<double>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                       (const cpx<double>*)src, M, nout, modes_of(p), tables_of(p), has_band, lo2,
+                       hi2, alpha, beta);
+  else
+    hipLaunchKernelGGL((k_divergence<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                       (const cpx<float>*)src, M, nout, modes_of(p), tables_of(p), has_band, lo2,
+                       hi2, (float)alpha, (float)beta);
+  return check_launch("divergence");
+}
+
+gbg_status op_curl(gbg_plan* p, void* dst, const void* src, int inverse, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  if (p->elem == 8) {
+    if (inverse)
+      hipLaunchKernelGGL((k_curl<double, true>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                         (const cpx<double>*)src, M, modes_of(p), tables_of(p));
+    else
+      hipLaunchKernelGGL((k_curl<double, false>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                         (const cpx<double>*)src, M, modes_of(p), tables_of(p));
+  } else {
+    if (inverse)
+      hipLaunchKernelGGL((k_curl<float, true>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                         (const cpx<float>*)src, M, modes_of(p), tables_of(p));
+    else
+      hipLaunchKernelGGL((k_curl<float, false>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                         (const cpx<float>*)src, M, modes_of(p), tables_of(p));
+  }
+  return check_launch("curl");
+}
+
+gbg_status op_laplacian(gbg_plan* p, void* dst, const void* src, int ncomp, int inverse,
+                        hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  if (p->elem == 8) {
+    if (inverse)
+      hipLaunchKernelGGL((k_laplacian<double, true>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                         (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p));
+    else
+      hipLaunchKernelGGL((k_laplacian<double, false>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                         (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p));
+  } else {
+    if (inverse)
+      hipLaunchKernelGGL((k_laplacian<float, true>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                         (const cpx<float>*)src, M, ncomp, modes_of(p), tables_of(p));
+    else
+      hipLaunchKernelGGL((k_laplacian<float, false>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                         (const cpx<float>*)src, M, ncomp, modes_of(p), tables_of(p));
+  }
+  return check_launch("laplacian");
+}
+
+gbg_status op_helmholtz(gbg_plan* p, void* dst, const void* src, int has_band, double lo,
+                        double hi, double alpha, double beta, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  const double lo2 = lo * lo, hi2 = hi * hi;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_helmholtz<double>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                       (const cpx<double>*)src, M, modes_of(p), tables_of(p), has_band, lo2, hi2,
+                       alpha, beta);
+  else
+    hipLaunchKernelGGL((k_helmholtz<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                       (const cpx<float>*)src, M, modes_of(p), tables_of(p), has_band, lo2, hi2,
+                       (float)alpha, (float)beta);
+  return check_launch("helmholtz");
+}
+
+gbg_status op_convolve_band(gbg_plan* p, void* dst, const void* src, int ncomp, double lo,
+                            double hi, double alpha, double beta, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  const double lo2 = lo * lo, hi2 = hi * hi;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_band<double>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+                       (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p), lo2, hi2,
+                       alpha, beta);
+  else
+    hipLaunchKernelGGL((k_band<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+                       (const cpx<float>*)src, M, ncomp, modes_of(p), tables_of(p), lo2, hi2,
+                       (float)alpha, (float)beta);
+  return check_launch("convolve band");
+}
+
+gbg_status op_mean_free(gbg_plan* p, void* spec, int ncomp, hipStream_t s) {
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_mean_free<double>), 1, 64, 0, s, (cpx<double>*)spec, p->M, ncomp);
+  else
+    hipLaunchKernelGGL((k_mean_free<float>), 1, 64, 0, s, (cpx<float>*)spec, p->M, ncomp);
+  return check_launch("mean free");
+}
+
+gbg_status op_cross(gbg_plan* p, void* dst, const void* u, const void* v, hipStream_t s) {
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_cross<double>), nblocks(p->P), kOpNT, 0, s, (double*)dst,
+                       (const double*)u, (const double*)v, p->P);
+  else
+    hipLaunchKernelGGL((k_cross<float>), nblocks(p->P), kOpNT, 0, s, (float*)dst, (const float*)u,
+                       (const float*)v, p->P);
+  return check_launch("cross");
+}
+
+gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hipStream_t s) {
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_multiply<double>), nblocks(p->P), kOpNT, 0, s, (double*)dst,
+                       (const double*)a, (const double*)b, p->P);
+  else
+    hipLaunchKernelGGL((k_multiply<float>), nblocks(p->P), kOpNT, 0, s, (float*)dst,
+                       (const float*)a, (const float*)b, p->P);
+  return check_launch("multiply");
+}
+
+gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int slot, hipStream_t s) {
+  const int nyq = (p->n[0] % 2 == 0) ? p->nh - 1 : -1;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_inner_part<double>), kRedBlocks, kOpNT, 0, s, (const cpx<double>*)f,
+                       (const cpx<double>*)g, p->M, ncomp, (unsigned)p->nh, nyq,
+                       p->red_part + (size_t)slot * kRedBlocks);
+  else
+    hipLaunchKernelGGL((k_inner_part<float>), kRedBlocks, kOpNT, 0, s, (const cpx<float>*)f,
+                       (const cpx<float>*)g, p->M, ncomp, (unsigned)p->nh, nyq,
+                       p->red_part + (size_t)slot * kRedBlocks);
+  gbg_status st = check_launch("inner partials");
+  if (st) return st;
+  const double N = (double)p->n[0] * (double)p->n[1] * (double)p->n[2];
+  hipLaunchKernelGGL(k_inner_final, 1, kOpNT, 0, s, p->red_part + (size_t)slot * kRedBlocks,
+                     kRedBlocks, N * N, p->red_out + slot);
+  return check_launch("inner final");
+}
+
+}  // namespace gbg

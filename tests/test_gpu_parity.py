@@ -1,0 +1,252 @@
+"""GPU parity: every entry point of libgbg against the FP64 oracle on the same
+seeded inputs, through the C ABI.
+
+Tolerances (relative L2 over stored entries): FP64 1e-12, FP32 1e-5 (the
+bars of the spectral path); the 2/3 mask is compared bit-exactly (the set of
+modes that are exactly zero)."""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import gbg_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-12, np.float32: 1e-5}
+GRIDS = [(8, 7, 6), (9, 9, 9), (7, 5, 3), (16, 12, 10), (32, 48, 40), (64, 64, 64), (30, 50, 36),
+         (1, 4, 6)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def G(gbg):
+    if gbg.device_count() < 1:
+        pytest.skip("no GPU")
+    return gbg
+
+
+def ogrid(n, L=(O.TWO_PI,) * 3):
+    return O.Grid3D(tuple(n), tuple(L))
+
+
+def rand_phys(g, order, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((3,) * order + g.physical_shape)
+
+
+def rand_spec(g, order, seed):
+    return O.to_spectral(rand_phys(g, order, seed))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", GRIDS)
+def test_transforms(G, n, dtype):
+    say(f"[transforms] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    for order in (0, 1):
+        f = rand_phys(g, order, 11 + order)
+        ref = O.to_spectral(f)
+        phys = G.Field(grid, "physical", order, dtype).upload(f)
+        spec = G.to_spectral(phys, plans)
+        got = spec.download()
+        assert O.rel_l2(got, ref) < TOL[dtype], "forward"
+        # inverse, 4-arg form preserves the source
+        back = G.to_physical(spec, plans)
+        assert O.rel_l2(back.download(), f) < 10 * TOL[dtype], "round trip"
+        np.testing.assert_array_equal(spec.download(), got)
+        # inverse of a non-Hermitian half spectrum follows the c2r convention
+        h = ref + 0.25 * np.conj(ref[..., ::-1, ::-1, :]) * 1j
+        hs = G.Field(grid, "spectral", order, dtype).upload(h)
+        out = G.Field(grid, "physical", order, dtype)
+        G.to_physical_(out, hs, plans)          # 3-arg: consumes hs
+        assert O.rel_l2(out.download(), O.to_physical(h, g)) < 10 * TOL[dtype], "c2r"
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,L", [((16, 12, 10), (O.TWO_PI,) * 3), ((8, 7, 6), (1.0, 2.5, 7.0)),
+                                 ((64, 64, 64), (O.TWO_PI,) * 3)])
+def test_operators(G, n, L, dtype):
+    say(f"[operators] n={n} L={L} dtype={np.dtype(dtype).name}")
+    g = ogrid(n, L)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n, L)
+    plans = G.FieldTransformPlans(grid, dtype)
+    tol = TOL[dtype]
+    s = rand_spec(g, 0, 1)
+    u = rand_spec(g, 1, 2)
+    A = rand_spec(g, 2, 3)
+    ds = G.Field(grid, "spectral", 0, dtype).upload(s)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dA = G.Field(grid, "spectral", 2, dtype).upload(A)
+
+    out = G.gradient(ds, plans).download()
+    assert O.rel_l2(out, O.gradient(s, m)) < tol
+    gt = G.gradient(du, plans).download()
+    ref_gt = np.stack([np.stack([(1j * a) * u[i] for a in (m.a1(), m.a2(), m.a3())]) for i in range(3)])
+    # tensor block (i, j) is stored at i + 3 j  ->  NumPy [j, i]
+    assert O.rel_l2(gt, np.swapaxes(ref_gt, 0, 1)) < tol
+
+    dv = G.Field(grid, "spectral", 0, dtype)
+    G.divergence_(dv, du, plans)
+    assert O.rel_l2(dv.download(), O.divergence(u, m)) < tol
+    band = G.spectral_band(plans, 1.5, 4.0)
+    pre = rand_spec(g, 0, 9)
+    dv.upload(pre)
+    G.divergence_(dv, du, plans, kernel=band, alpha=0.5, beta=2.0)
+    ref = 0.5 * O.band_mask(m, 1.5, 4.0) * O.divergence(u, m) + 2.0 * pre
+    assert O.rel_l2(dv.download(), ref) < tol
+    dvec = G.Field(grid, "spectral", 1, dtype)
+    G.divergence_(dvec, dA, plans)
+    # tensor->vector: out_i = i sum_j a_j A_{ij}; A block (i,j) = NumPy [j, i]
+    ref = np.stack([1j * ((m.a1() * A[0, i] + m.a2() * A[1, i]) + m.a3() * A[2, i]) for i in range(3)])
+    assert O.rel_l2(dvec.download(), ref) < tol
+
+    c = G.curl(du, plans).download()
+    assert O.rel_l2(c, O.curl(u, m)) < tol
+    alias = G.Field(grid, "spectral", 1, dtype).upload(u)
+    G.curl_(alias, alias, plans)
+    np.testing.assert_array_equal(alias.download(), c)          # aliased == non-aliased, bitwise
+    ic = G.Field(grid, "spectral", 1, dtype)
+    G.inverse_curl_(ic, du, plans)
+    assert O.rel_l2(ic.download(), O.inverse_curl(u, m)) < tol
+
+    lap = G.Field(grid, "spectral", 1, dtype)
+    G.laplacian_(lap, du, plans)
+    assert O.rel_l2(lap.download(), O.laplacian(u, m)) < tol
+    G.inverse_laplacian_(lap, du, plans)
+    assert O.rel_l2(lap.download(), O.inverse_laplacian(u, m)) < tol
+
+    hp = G.Field(grid, "spectral", 1, dtype)
+    G.helmholtz_projection_(hp, du, plans)
+    assert O.rel_l2(hp.download(), O.helmholtz_projection(u, m)) < tol
+    pre3 = rand_spec(g, 1, 8)
+    hp.upload(pre3)
+    G.helmholtz_projection_(hp, du, plans, kernel=band, alpha=0.3, beta=1.0)
+    ref = O.helmholtz_projection(u, m, O.band_mask(m, 1.5, 4.0), 0.3, 1.0, pre3)
+    assert O.rel_l2(hp.download(), ref) < tol
+
+    cv = G.Field(grid, "spectral", 1, dtype)
+    kc = G.dealias_cutoff(n[0], L[0])
+    G.convolve_(cv, du, G.spectral_cutoff(plans, kc), plans)
+    got = cv.download()
+    refm = O.band_mask(m, 0.0, kc)
+    assert O.rel_l2(got, refm[None] * u) < tol
+    assert np.array_equal(got[:, refm == 0] == 0, np.ones_like(got[:, refm == 0], bool))
+
+    mf = G.Field(grid, "spectral", 1, dtype).upload(u)
+    G.mean_free_projection_(mf, plans)
+    assert O.rel_l2(mf.download(), O.mean_free(u)) < tol
+
+    # physical products
+    pu, pv = rand_phys(g, 1, 5), rand_phys(g, 1, 6)
+    fu = G.Field(grid, "physical", 1, dtype).upload(pu)
+    fv = G.Field(grid, "physical", 1, dtype).upload(pv)
+    fx = G.Field(grid, "physical", 1, dtype)
+    G.cross_(fx, fu, fv, plans)
+    assert O.rel_l2(fx.download(), O.cross(pu.astype(dtype), pv.astype(dtype))) < tol
+    a0, b0 = rand_phys(g, 0, 7), rand_phys(g, 0, 8)
+    fa = G.Field(grid, "physical", 0, dtype).upload(a0)
+    fb = G.Field(grid, "physical", 0, dtype).upload(b0)
+    G.multiply_(fa, fa, fb, plans)
+    assert O.rel_l2(fa.download(), (a0.astype(dtype) * b0.astype(dtype))) < tol
+
+    # Parseval inner (FP64 accumulation)
+    ip = G.inner(du, G.Field(grid, "spectral", 1, dtype).upload(pre3), plans)
+    ref = O.inner_spectral(u.astype(plans.cdtype).astype(np.complex128),
+                           pre3.astype(plans.cdtype).astype(np.complex128), g)
+    assert abs(ip - ref) <= 1e-12 * abs(ref) + 1e-300
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(16, 16, 16), (64, 64, 64), (48, 32, 40)])
+@pytest.mark.parametrize("fused", [False, True])
+def test_uxw_parity(G, n, dtype, fused):
+    say(f"[uxw] n={n} dtype={np.dtype(dtype).name} fused={fused}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = O.synthetic_field(g, 2001, vector=True, m=m)
+    kc = O.dealias_cutoff(n[0])
+    ref, E, Z = O.uxw(u, g, kc, m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dn = G.Field(grid, "spectral", 1, dtype)
+    e, z = G.uxw_(dn, du, plans, kc, fused=fused)
+    got = dn.download()
+    tol = TOL[dtype]
+    assert O.rel_l2(got, ref) < tol
+    assert abs(e - E) <= tol * abs(E) and abs(z - Z) <= tol * abs(Z)
+    # the mask, bit-exactly: every mode outside it is exactly zero
+    mask = O.band_mask(m, 0.0, kc)
+    assert np.all(got[:, mask == 0] == 0)
+    assert np.all(got[:, 0, 0, 0] == 0)
+    # input untouched
+    assert np.array_equal(du.download(), u.astype(plans.cdtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_config1_pipeline(G, dtype):
+    """64^3 plain method: rfft of a and b, gradient of a, dealiased a*b, irfft."""
+    say(f"[config1] dtype={np.dtype(dtype).name}")
+    n = (64, 64, 64)
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    a_hat = O.synthetic_field(g, 1001, m=m)
+    b_hat = O.synthetic_field(g, 1002, m=m)
+    a = O.to_physical(a_hat, g)
+    b = O.to_physical(b_hat, g)
+    kc = O.dealias_cutoff(64)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    tol = TOL[dtype]
+    fa = G.Field(grid, "physical", 0, dtype).upload(a)
+    fb = G.Field(grid, "physical", 0, dtype).upload(b)
+    ah = G.to_spectral(fa, plans)
+    bh = G.to_spectral(fb, plans)
+    assert O.rel_l2(ah.download(), O.to_spectral(a)) < tol
+    assert O.rel_l2(bh.download(), O.to_spectral(b)) < tol
+    gh = G.gradient(ah, plans)
+    ref_gh = O.gradient(O.to_spectral(a), m)
+    assert O.rel_l2(gh.download(), ref_gh) < tol
+    gp = G.to_physical(gh, plans)
+    assert O.rel_l2(gp.download(), O.to_physical(ref_gh, g)) < tol
+    ph = G.Field(grid, "spectral", 0, dtype)
+    G.dealiased_product_(ph, ah, bh, plans, kc)
+    ref_ph = O.dealiased_product(O.to_spectral(a), O.to_spectral(b), g, kc, m)
+    got = ph.download()
+    assert O.rel_l2(got, ref_ph) < tol
+    mask = O.band_mask(m, 0.0, kc)
+    assert np.all(got[mask == 0] == 0)
+    pp = G.to_physical(ph, plans)
+    assert O.rel_l2(pp.download(), O.to_physical(ref_ph, g)) < tol
+
+
+def test_refusals(G):
+    say("[refusals]")
+    grid = G.Grid3D((8, 8, 8))
+    plans = G.FieldTransformPlans(grid)
+    with pytest.raises(G.UnsupportedError):
+        G.FieldTransformPlans(G.Grid3D((17, 8, 8)))
+    with pytest.raises(G.ArgumentError):
+        G.FieldTransformPlans(grid, np.int32)
+    s = G.Field(grid, "spectral", 1)
+    with pytest.raises(G.AliasError):
+        G.to_physical_(G.Field(grid, "physical", 1), s, plans, scratch=s)
+    with pytest.raises(G.DimensionMismatch):
+        G.inner(G.Field(grid, "spectral", 0), s, plans)
+    with pytest.raises(G.ArgumentError):
+        G.spectral_band(plans, 3.0, 1.0)
+    other = G.FieldTransformPlans(G.Grid3D((8, 8, 4)))
+    with pytest.raises(G.ArgumentError):
+        G.curl_(s, s, other)
+    with pytest.raises(G.UnsupportedError):
+        G.curl_(G.Field(grid, "physical", 1), s, plans)

@@ -1,0 +1,275 @@
+"""Pins the FP64 oracle against closed forms, brute force and the reference's
+own known-answer tests (CPU only).  Each check is chosen so that a plausible
+mistake -- a dropped term, a wrong sign or index, a transposed axis -- fails
+one of them."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import gbg_oracle as O
+
+TWO_PI = 2 * math.pi
+
+
+def phys_field(grid, fn):
+    Nx, Ny, Nz = grid.n
+    x = np.arange(Nx) * grid.L[0] / Nx
+    y = np.arange(Ny) * grid.L[1] / Ny
+    z = np.arange(Nz) * grid.L[2] / Nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return fn(X, Y, Z)
+
+
+# --- wavenumber tables: test/fields.jl:121-145 ------------------------------
+
+@pytest.mark.parametrize("n,L", [(8, TWO_PI), (7, TWO_PI), (6, 3.0), (9, 1.0), (512, TWO_PI)])
+def test_wavenumbers_vs_fftfreq(n, L):
+    unit = TWO_PI / L
+    kd_h, ks_h = O.axis_wavenumber_values(n, L, True)
+    kd_f, ks_f = O.axis_wavenumber_values(n, L, False)
+    kr = unit * np.fft.rfftfreq(n, 1.0 / n)
+    kf = unit * np.fft.fftfreq(n, 1.0 / n)
+    np.testing.assert_allclose(ks_h, kr ** 2, rtol=1e-15)
+    np.testing.assert_allclose(ks_f, kf ** 2, rtol=1e-15)
+    kd_h_ref, kd_f_ref = kr.copy(), kf.copy()
+    if n % 2 == 0:
+        kd_h_ref[-1] = 0.0
+        kd_f_ref[n // 2] = 0.0
+    np.testing.assert_allclose(kd_h, kd_h_ref, rtol=1e-15)
+    np.testing.assert_allclose(kd_f, kd_f_ref, rtol=1e-15)
+    if n % 2 == 0:   # the Nyquist split: derivative zero, squared keeps k^2
+        assert kd_f[n // 2] == 0.0 and ks_f[n // 2] == (unit * (n // 2)) ** 2
+
+
+# --- transforms: brute force, round trip ------------------------------------
+
+@pytest.mark.parametrize("n", [(8, 7, 6), (9, 9, 9), (7, 5, 3), (6, 4, 5)])
+def test_transform_vs_direct_dft(n):
+    g = O.Grid3D(n)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(g.physical_shape)
+    F = O.to_spectral(f)
+    np.testing.assert_allclose(F, O.direct_dft_forward(f), rtol=0, atol=1e-11)
+    # inverse on a NON-Hermitian half spectrum pins the c2r convention
+    H = F + 0.3j * rng.standard_normal(F.shape)
+    np.testing.assert_allclose(O.to_physical(H, g), O.direct_dft_inverse(H, g), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [(8, 8, 8), (8, 7, 6), (7, 5, 3), (9, 9, 9), (16, 12, 10)])
+def test_round_trip(n):
+    g = O.Grid3D(n)
+    f = np.random.default_rng(1).standard_normal((3,) + g.physical_shape)
+    back = O.to_physical(O.to_spectral(f), g)
+    assert O.rel_l2(back, f) < 1e-12      # test/fields.jl:204 rtol 1e-12
+
+
+# --- operators: closed forms (test/differentialoperators.jl) -----------------
+
+def _grid16():
+    return O.Grid3D((16, 16, 16))
+
+
+def test_gradient_sin_cos():
+    g = _grid16()
+    m = O.Multipliers.build(g)
+    s = O.to_spectral(phys_field(g, lambda x, y, z: np.sin(x) + np.cos(2 * y) * np.sin(3 * z)))
+    gr = O.to_physical(O.gradient(s, m), g)
+    np.testing.assert_allclose(gr[0], phys_field(g, lambda x, y, z: np.cos(x)), atol=1e-12)
+    np.testing.assert_allclose(gr[1], phys_field(g, lambda x, y, z: -2 * np.sin(2 * y) * np.sin(3 * z)), atol=1e-12)
+    np.testing.assert_allclose(gr[2], phys_field(g, lambda x, y, z: 3 * np.cos(2 * y) * np.cos(3 * z)), atol=1e-12)
+
+
+def test_curl_of_shear_and_divergence():
+    g = _grid16()
+    m = O.Multipliers.build(g)
+    u = np.stack([phys_field(g, lambda x, y, z: 0 * x), phys_field(g, lambda x, y, z: 0 * x),
+                  phys_field(g, lambda x, y, z: np.sin(x))])
+    c = O.to_physical(O.curl(O.to_spectral(u), m), g)
+    np.testing.assert_allclose(c[0], 0, atol=1e-12)
+    np.testing.assert_allclose(c[1], -phys_field(g, lambda x, y, z: np.cos(x)), atol=1e-12)
+    np.testing.assert_allclose(c[2], 0, atol=1e-12)
+    # asymmetric divergence: d/dx sin(y) + d/dy cos(2x+y) + d/dz sin(3z)
+    A = np.stack([phys_field(g, lambda x, y, z: np.sin(x + y)),
+                  phys_field(g, lambda x, y, z: np.cos(2 * x + y)),
+                  phys_field(g, lambda x, y, z: np.sin(3 * z))])
+    d = O.to_physical(O.divergence(O.to_spectral(A), m), g)
+    ref = phys_field(g, lambda x, y, z: np.cos(x + y) - np.sin(2 * x + y) + 3 * np.cos(3 * z))
+    np.testing.assert_allclose(d, ref, atol=1e-12)
+
+
+def test_laplacian_inverse_and_nyquist_split():
+    g = O.Grid3D((8, 8, 8))
+    m = O.Multipliers.build(g)
+    phi = O.to_spectral(phys_field(g, lambda x, y, z: np.cos(4 * x)))
+    grad = O.to_physical(O.gradient(phi, m), g)
+    assert np.max(np.abs(grad)) < 1e-12                       # derivative zeroed at Nyquist
+    lap = O.to_physical(O.laplacian(phi, m), g)
+    np.testing.assert_allclose(lap, -16 * phys_field(g, lambda x, y, z: np.cos(4 * x)), atol=1e-12)
+    f = O.to_spectral(phys_field(g, lambda x, y, z: np.sin(x) * np.cos(2 * y) + np.sin(3 * z)))
+    back = O.inverse_laplacian(O.laplacian(f, m), m)
+    np.testing.assert_allclose(back, f, atol=1e-12)
+    lap_sin = O.to_physical(O.laplacian(O.to_spectral(phys_field(g, lambda x, y, z: np.sin(x))), m), g)
+    np.testing.assert_allclose(lap_sin, -phys_field(g, lambda x, y, z: np.sin(x)), atol=1e-12)
+
+
+def test_helmholtz_projector_properties():
+    g = O.Grid3D((8, 7, 6))
+    m = O.Multipliers.build(g)
+    rng = np.random.default_rng(3)
+    u = O.to_spectral(rng.standard_normal((3,) + g.physical_shape))
+    p = O.helmholtz_projection(u, m)
+    assert np.max(np.abs(O.divergence(p, m))) < 1e-12 * np.max(np.abs(u))     # solenoidal
+    np.testing.assert_allclose(O.helmholtz_projection(p, m), p, atol=1e-13)   # idempotent
+    # gradient fields are annihilated
+    s = O.to_spectral(rng.standard_normal(g.physical_shape))
+    assert np.max(np.abs(O.helmholtz_projection(O.gradient(s, m), m))) < 1e-11
+
+
+def test_inverse_curl_roundtrip():
+    g = O.Grid3D((8, 8, 6))
+    m = O.Multipliers.build(g)
+    rng = np.random.default_rng(4)
+    w = O.helmholtz_projection(O.to_spectral(rng.standard_normal((3,) + g.physical_shape)), m)
+    # modes whose derivative wavenumber vanishes (k=0 and the all-Nyquist/zero
+    # corners) have no preimage under curl; drop them
+    sq = (m.a1() ** 2 + m.a2() ** 2) + m.a3() ** 2
+    w = w * (sq != 0)[None]
+    # curl(inverse_curl(w)) == w at every mode for a solenoidal w
+    np.testing.assert_allclose(O.curl(O.inverse_curl(w, m), m), w, atol=1e-12)
+
+
+# --- mask: test/convolution.jl:190-227 --------------------------------------
+
+@pytest.mark.parametrize("N", [8, 16, 64])
+def test_mask_integer_predicate_and_idempotent(N):
+    g = O.Grid3D((N, N, N))
+    m = O.Multipliers.build(g)
+    kc = O.dealias_cutoff(N)
+    M = O.band_mask(m, 0.0, kc)
+    # on a 2pi box k are integers: the mask is |n|^2 <= floor(kc^2)
+    nx = np.arange(g.nh)
+    ny = np.where(np.arange(N) < (N + 1) // 2, np.arange(N), np.arange(N) - N)
+    n2 = (nx[None, None, :] ** 2 + ny[None, :, None] ** 2) + ny[:, None, None] ** 2
+    ref = (n2 <= math.floor(kc * kc)).astype(float)
+    assert np.array_equal(M, ref)
+    assert M[0, 0, 0] == 1.0 and (M == 0).any()
+    assert set(np.unique(M)) <= {0.0, 1.0}
+    assert np.array_equal(M * M, M)
+
+
+def test_band_membership_and_errors():
+    g = O.Grid3D((8, 8, 8))
+    m = O.Multipliers.build(g)
+    b = O.band_mask(m, 1.5, 3.0)
+    ks = m.ksq()
+    assert np.array_equal(b, ((1.5 ** 2 <= ks) & (ks <= 3.0 ** 2)).astype(float))
+    assert b[0, 0, 0] == 0.0
+    with pytest.raises(ValueError):
+        O.band_mask(m, 3.0, 1.0)
+    with pytest.raises(ValueError):
+        O.band_mask(m, -1.0, 1.0)
+
+
+def test_mask_golden_64():
+    """Kept-mode count at 64^3 and 512^3 (closed form count of integer
+    lattice points with |n|^2 <= floor(kc^2), n_x >= 0)."""
+    for N in (64, 512):
+        kc2 = math.floor(O.dealias_cutoff(N) ** 2)
+        r = math.isqrt(kc2)
+        cnt = 0
+        for nx in range(0, r + 1):
+            for ny in range(-r, r + 1):
+                rem = kc2 - nx * nx - ny * ny
+                if rem >= 0:
+                    cnt += 2 * math.isqrt(rem) + 1
+        if N == 64:
+            g = O.Grid3D((N, N, N))
+            assert int(O.band_mask(O.Multipliers.build(g), 0.0, O.dealias_cutoff(N)).sum()) == cnt
+
+
+# --- Parseval: test/reductions.jl:144-172 ------------------------------------
+
+@pytest.mark.parametrize("n", [(8, 8, 8), (7, 8, 6)])
+def test_parseval_known_answers(n):
+    g = O.Grid3D(n)
+    f1 = phys_field(g, lambda x, y, z: np.sin(x) + np.cos(2 * y) * np.sin(z))
+    f2 = phys_field(g, lambda x, y, z: 2 * np.sin(x) - np.cos(2 * y) * np.sin(z))
+    assert abs(O.inner_spectral(O.to_spectral(f1), O.to_spectral(f2), g) - 0.75) < 1e-12
+    assert abs(O.inner_physical(f1, f2, g) - 0.75) < 1e-12
+    u1 = np.stack([phys_field(g, lambda x, y, z: np.sin(x)), phys_field(g, lambda x, y, z: np.cos(y)),
+                   phys_field(g, lambda x, y, z: np.sin(2 * z))])
+    u2 = np.stack([phys_field(g, lambda x, y, z: 2 * np.sin(x)), phys_field(g, lambda x, y, z: np.cos(y)),
+                   phys_field(g, lambda x, y, z: np.cos(z))])
+    assert abs(O.inner_spectral(O.to_spectral(u1), O.to_spectral(u2), g) - 1.5) < 1e-12
+    r = np.random.default_rng(2).standard_normal((3,) + g.physical_shape)
+    assert abs(O.inner_spectral(O.to_spectral(r), O.to_spectral(r), g) - O.inner_physical(r, r, g)) < 1e-12
+
+
+def test_mean_free_vs_physical_average():
+    g = O.Grid3D((8, 7, 6))
+    f = np.random.default_rng(5).standard_normal(g.physical_shape) + 0.7
+    back = O.to_physical(O.mean_free(O.to_spectral(f)), g)
+    np.testing.assert_allclose(back, f - f.mean(), atol=16 * np.finfo(float).eps * np.max(np.abs(f)))
+
+
+# --- pipelines: closed forms ---------------------------------------------------
+
+def test_uxw_beltrami_vanishes():
+    """ABC flow: curl u = u, so u x w = 0 and N^ = 0."""
+    g = O.Grid3D((16, 16, 16))
+    A, B, C = 1.0, 0.7, 0.4
+    u = np.stack([phys_field(g, lambda x, y, z: A * np.sin(z) + C * np.cos(y)),
+                  phys_field(g, lambda x, y, z: B * np.sin(x) + A * np.cos(z)),
+                  phys_field(g, lambda x, y, z: C * np.sin(y) + B * np.cos(x))])
+    nh, E, Z = O.uxw(O.to_spectral(u), g)
+    assert np.max(np.abs(nh)) < 1e-9
+    # E = 1/2 <u,u> = 1/2 (A^2 + B^2 + C^2) for the ABC flow; Z = E (Beltrami)
+    assert abs(E - 0.5 * (A * A + B * B + C * C)) < 1e-12
+    assert abs(Z - E) < 1e-12
+
+
+def test_uxw_shear_pair_closed_form():
+    """u = (sin z, sin x, 0): w = (0, cos z, cos x); u x w = (sin x cos x,
+    -sin z cos x, sin z cos z); its solenoidal part is (0, -sin z cos x, 0)."""
+    g = O.Grid3D((16, 16, 16))
+    u = np.stack([phys_field(g, lambda x, y, z: np.sin(z)), phys_field(g, lambda x, y, z: np.sin(x)),
+                  phys_field(g, lambda x, y, z: 0 * x)])
+    nh, E, Z = O.uxw(O.to_spectral(u), g)
+    n = O.to_physical(nh, g)
+    np.testing.assert_allclose(n[0], 0, atol=1e-12)
+    np.testing.assert_allclose(n[1], -phys_field(g, lambda x, y, z: np.sin(z) * np.cos(x)), atol=1e-12)
+    np.testing.assert_allclose(n[2], 0, atol=1e-12)
+    assert abs(E - 0.5) < 1e-12 and abs(Z - 0.5) < 1e-12
+
+
+def test_dealiased_product_removes_alias():
+    """cos(k1 x) cos(k2 x) with k1 + k2 > N/2: the 2/3 rule keeps only
+    1/2 cos((k1-k2) x); without truncation the sum mode would alias back."""
+    N = 24
+    g = O.Grid3D((N, N, N))
+    kc = O.dealias_cutoff(N)            # 8
+    k1, k2 = 8, 7
+    a = O.to_spectral(phys_field(g, lambda x, y, z: np.cos(k1 * x)))
+    b = O.to_spectral(phys_field(g, lambda x, y, z: np.cos(k2 * x)))
+    p = O.to_physical(O.dealiased_product(a, b, g, kc), g)
+    np.testing.assert_allclose(p, 0.5 * phys_field(g, lambda x, y, z: np.cos((k1 - k2) * x)), atol=1e-12)
+    # a mode above kc in an input is truncated away
+    c = O.to_spectral(phys_field(g, lambda x, y, z: np.cos(9 * y)))
+    assert np.max(np.abs(O.dealiased_product(c, b, g, kc))) < 1e-12 * g.npoints  # spectral scale N^3
+
+
+def test_synthetic_field_properties():
+    g = O.Grid3D((16, 12, 10))
+    m = O.Multipliers.build(g)
+    u = O.synthetic_field(g, 2001, vector=True, m=m)
+    assert abs(O.inner_spectral(u, u, g) - 3.0) < 1e-12
+    assert np.max(np.abs(O.divergence(u, m))) < 1e-12
+    assert np.all(u[:, 0, 0, 0] == 0)
+    # Hermitian on the kx=0 plane: physical field is real => round trip exact
+    back = O.to_spectral(O.to_physical(u, g))
+    assert O.rel_l2(back, u) < 1e-12
+    # seeded: deterministic and seed-sensitive
+    assert np.array_equal(O.synthetic_field(g, 2001, True, m), u)
+    assert not np.array_equal(O.synthetic_field(g, 2002, True, m), u)
