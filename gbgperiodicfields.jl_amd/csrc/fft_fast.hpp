@@ -1,0 +1,208 @@
+// Register-resident pencil FFT for power-of-two lengths (compile-time N).
+//
+// A pencil of length N is spread over TP = N/E threads; thread t holds the E
+// elements x[t + TP*m], m = 0..E-1, in registers.  The transform is the
+// radix-8 Stockham autosort factorisation (radix 8 while N/Ls allows, then
+// one closing radix 2 or 4 stage):
+//   stage (R, Ls), butterfly j in [0, N/R), k = j mod Ls:
+//     v_r = x[j + r N/R] * w_{Ls R}^{r k};  V = DFT_R(v);  y[(j-k) R + k + r Ls] = V_r
+// With j = t + TP*b the first stage reads exactly the registers a thread
+// already holds and the last stage writes back to the same pattern, so a
+// pencil is loaded from and stored to global memory directly in that
+// pattern; only the stages in between exchange through LDS (one write, one
+// barrier, one read, one barrier per exchange).  Twiddles come from an
+// N-entry table e^{-2 pi i m/N} computed in extended precision on the host:
+// w^1, w^2, w^4 are read and w^3, w^5, w^6, w^7 formed by one product each.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace gbg {
+namespace fast {
+
+// wave-scope barrier: orders the wave's LDS writes before its later reads
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// SYNC = 0: the pencil's threads span several waves (workgroup barrier);
+// SYNC = 1: all of a pencil's threads sit in one wave.
+template <int SYNC>
+__device__ __forceinline__ void lds_sync() {
+  if (SYNC)
+    wave_sync();
+  else
+    __syncthreads();
+}
+
+template <int N, int LS>
+struct StageR {
+  static constexpr int rem = N / LS;
+  static constexpr int R = (rem % 8 == 0) ? 8 : rem;
+};
+
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> tw_load(const cpx<T>* __restrict__ tw, int m) {
+  cpx<T> w = tw[m];
+  if (DIR > 0) w.y = -w.y;
+  return w;
+}
+
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void twiddle(cpx<T>* a, int k, int step, const cpx<T>* __restrict__ tw) {
+  if constexpr (R == 8) {
+    const int m = k * step;
+    const cpx<T> w1 = tw_load<DIR>(tw, m), w2 = tw_load<DIR>(tw, 2 * m), w4 = tw_load<DIR>(tw, 4 * m);
+    const cpx<T> w3 = cmul(w2, w1);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], w3);
+    a[4] = cmul(a[4], w4);
+    a[5] = cmul(a[5], cmul(w4, w1));
+    a[6] = cmul(a[6], cmul(w4, w2));
+    a[7] = cmul(a[7], cmul(w4, w3));
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw_load<DIR>(tw, r * k * step));
+  }
+}
+
+template <typename T, int N, int E, int LS, int DIR, int SYNC>
+__device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* pen, int t,
+                                       const cpx<T>* __restrict__ tw) {
+  constexpr int R = StageR<N, LS>::R;
+  static_assert(E % R == 0, "elements per thread must be a multiple of each radix");
+  constexpr int NB = E / R;
+  constexpr int NR = N / R;
+  constexpr int TP = N / E;
+  constexpr bool FIRST = (LS == 1);
+  constexpr bool LAST = (LS * R == N);
+  cpx<T> a[NB][R];
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[b][r] = v[b + r * NB];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + TP * b;
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[b][r] = pen[j + r * NR];
+    }
+    lds_sync<SYNC>();
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if constexpr (!FIRST) {
+      const int j = t + TP * b;
+      twiddle<T, R, DIR>(a[b], j % LS, N / (LS * R), tw);
+    }
+    Dft<R, DIR, T>::run(a[b], tw, N);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = a[b][r];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + TP * b;
+      const int k = j % LS;
+      cpx<T>* o = pen + (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) o[r * LS] = a[b][r];
+    }
+    lds_sync<SYNC>();
+    stages<T, N, E, LS * R, DIR, SYNC>(v, pen, t, tw);
+  }
+}
+
+// In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  `pen` is
+// this pencil's LDS region (N elements); it is free again on return.
+template <typename T, int N, int E, int DIR, int SYNC>
+__device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* pen, int t, const cpx<T>* __restrict__ tw) {
+  stages<T, N, E, 1, DIR, SYNC>(v, pen, t, tw);
+}
+
+// LDS pencil stride: padding spreads the pencils of a tile over the banks.
+template <typename T, int N>
+struct Ld {
+  static constexpr int value = N + (sizeof(T) == 8 ? 2 : 4);
+};
+
+// per-length launch shape: E elements per thread, TP threads per pencil,
+// B pencils per workgroup of the strided passes
+template <typename T, int N>
+struct Cfg {
+  static constexpr int E = 8;
+  static constexpr int TP = N / E;
+  static constexpr int WG = sizeof(T) == 8 ? 512 : 1024;
+  static constexpr int B0 = WG / TP;
+  static constexpr int B = B0 < 1 ? 1 : (B0 > 64 ? 64 : B0);
+  static constexpr int NT = B * TP;
+  static constexpr int LD = Ld<T, N>::value;
+  // x pass: row-pair units per workgroup
+  static constexpr int U = TP >= 256 ? 1 : 256 / TP;
+  static constexpr int XNT = U * TP;
+  static constexpr int XSYNC = TP <= 64 ? 1 : 0;
+};
+
+
+// Row-pair helpers of the x passes.  G provides nh (stored kx count of the
+// full grid), nhk (kx indices actually held, <= nh) and nyq_x (the even-axis
+// Nyquist index, or -1).
+//
+// c2r_pair_load: Z_k = A_k + i B_k over the Hermitian completion of the two
+// half-spectrum rows ra, rb (entries kx >= nhk are zero), with the
+// self-conjugate entries taken real as a c2r transform treats them.
+template <typename T, int N, int E, class G>
+__device__ __forceinline__ void c2r_pair_load(cpx<T> (&v)[E], const cpx<T>* __restrict__ ra,
+                                              const cpx<T>* __restrict__ rb, bool oka, bool okb,
+                                              int t, const G& g) {
+  using C = Cfg<T, N>;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + C::TP * m;
+    const bool up = k >= g.nh;
+    const int kk = up ? N - k : k;
+    cpx<T> a = mk<T>(T(0), T(0)), b = mk<T>(T(0), T(0));
+    if (kk < g.nhk) {
+      if (oka) a = ra[kk];
+      if (okb) b = rb[kk];
+    }
+    if (kk == 0 || kk == g.nyq_x) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    if (up) {
+      a.y = -a.y;
+      b.y = -b.y;
+    }
+    v[m] = mk<T>(a.x - b.y, a.y + b.x);
+  }
+}
+
+// forward-transformed Z = a + i b -> A_k, B_k for k < nhk, stored to ra / rb
+template <typename T, int N, int E, class G>
+__device__ __forceinline__ void r2c_pair_store(cpx<T> (&v)[E], cpx<T>* pen,
+                                               cpx<T>* __restrict__ ra, cpx<T>* __restrict__ rb,
+                                               bool oka, bool okb, int t, const G& g) {
+  using C = Cfg<T, N>;
+#pragma unroll
+  for (int m = 0; m < E; ++m) pen[t + C::TP * m] = v[m];
+  lds_sync<C::XSYNC>();
+  for (int k = t; k < g.nhk; k += C::TP) {
+    const cpx<T> z = pen[k];
+    const cpx<T> zc = pen[k == 0 ? 0 : N - k];
+    if (oka) ra[k] = mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y));
+    if (okb) rb[k] = mk<T>(T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x));
+  }
+  lds_sync<C::XSYNC>();
+}
+
+}  // namespace fast
+}  // namespace gbg

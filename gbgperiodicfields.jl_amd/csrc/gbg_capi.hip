@@ -2,6 +2,7 @@
 // reference's refusals), plan construction, device memory, and dispatch to
 // the kernel launchers.  No C++ exception crosses this boundary.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -181,6 +182,10 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
     }
   }
   p->elem = elem;
+  {
+    const char* env = std::getenv("GBG_GENERIC_FFT");
+    p->generic_only = (env && env[0] == '1') ? 1 : 0;
+  }
   p->nh = n[0] / 2 + 1;
   p->M = M;
   p->P = (size_t)n[0] * n[1] * n[2];

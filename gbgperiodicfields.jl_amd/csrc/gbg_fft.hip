@@ -6,7 +6,7 @@
 // Strided passes load B consecutive pencils of the fast index into LDS, so
 // every global row segment is B contiguous complex elements; the x passes
 // pair two real rows into one complex transform (z = a + i b).
-#include "fft_core.hpp"
+#include "fft_fast.hpp"
 
 namespace gbg {
 
@@ -112,6 +112,148 @@ __global__ __launch_bounds__(kNT) void k_c2r_x(const cpx<T>* __restrict__ in, T*
 }
 
 // ----------------------------------------------------------------------------
+// register-resident passes for power-of-two lengths (fft_fast.hpp)
+// ----------------------------------------------------------------------------
+using fast::Cfg;
+
+// strided C2C pass: B pencils per workgroup, thread (pl, t) holds elements
+// t + TP*m of pencil pl; consecutive threads read consecutive pencils, so
+// every global access is a B-element contiguous segment.  In place safe:
+// every load of the tile precedes the first barrier, every store follows
+// the last one.
+template <typename T, int N, int DIR>
+__global__ __launch_bounds__((Cfg<T, N>::NT)) void k_fast_c2c(const cpx<T>* in, cpx<T>* out,
+                                                            Strided d, const cpx<T>* __restrict__ tw,
+                                                            T scale) {
+  using C = Cfg<T, N>;
+  constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
+  __shared__ cpx<T> lds[B * LD];
+  const int pl = threadIdx.x % B, t = threadIdx.x / B;
+  const long long p = (long long)blockIdx.x * B + pl;
+  const bool valid = p < d.npencil;
+  const long long base =
+      (long long)blockIdx.y * d.cstride + (valid ? (p / d.inner) * d.ostride + (p % d.inner) : 0);
+  cpx<T> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    v[m] = valid ? in[base + (long long)(t + TP * m) * d.se] : mk<T>(T(0), T(0));
+  fast::fft<T, N, E, DIR, 0>(v, lds + pl * LD, t, tw);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) out[base + (long long)(t + TP * m) * d.se] = cscale(v[m], scale);
+  }
+}
+
+struct RowG {
+  int nh, nhk, nyq_x;
+};
+
+// x-pass R2C over row pairs (z = a + i b), one pair per TP threads
+template <typename T, int N>
+__global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __restrict__ in,
+                                                               cpx<T>* __restrict__ out,
+                                                               long long nrows, RowG g,
+                                                               const cpx<T>* __restrict__ tw) {
+  using C = Cfg<T, N>;
+  constexpr int E = C::E, TP = C::TP, LD = C::LD;
+  __shared__ cpx<T> lds[C::U * LD];
+  const int un = threadIdx.x / TP, t = threadIdx.x % TP;
+  const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  cpx<T> v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int x = t + TP * m;
+    v[m] = mk<T>(ok0 ? in[r0 * N + x] : T(0), ok1 ? in[r1 * N + x] : T(0));
+  }
+  cpx<T>* pen = lds + un * LD;
+  fast::fft<T, N, E, -1, C::XSYNC>(v, pen, t, tw);
+  fast::r2c_pair_store<T, N, E>(v, pen, out + (ok0 ? r0 * g.nh : 0), out + (ok1 ? r1 * g.nh : 0),
+                                ok0, ok1, t, g);
+}
+
+// x-pass C2R over row pairs; output scaled by `scale`
+template <typename T, int N>
+__global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_c2r_x(const cpx<T>* __restrict__ in,
+                                                               T* __restrict__ out, long long nrows,
+                                                               RowG g, const cpx<T>* __restrict__ tw,
+                                                               T scale) {
+  using C = Cfg<T, N>;
+  constexpr int E = C::E, TP = C::TP, LD = C::LD;
+  __shared__ cpx<T> lds[C::U * LD];
+  const int un = threadIdx.x / TP, t = threadIdx.x % TP;
+  const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
+  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  cpx<T> v[E];
+  fast::c2r_pair_load<T, N, E>(v, in + (ok0 ? r0 * g.nh : 0), in + (ok1 ? r1 * g.nh : 0), ok0, ok1,
+                               t, g);
+  fast::fft<T, N, E, 1, C::XSYNC>(v, lds + un * LD, t, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int x = t + TP * m;
+    if (ok0) out[r0 * N + x] = v[m].x * scale;
+    if (ok1) out[r1 * N + x] = v[m].y * scale;
+  }
+}
+
+static bool fast_len(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
+}
+
+template <typename T, int N>
+static void launch_fast_c2c(int dir, const cpx<T>* in, cpx<T>* out, const Strided& d, int ncomp,
+                            const cpx<T>* tw, T scale, hipStream_t s) {
+  using C = Cfg<T, N>;
+  dim3 grid((unsigned)((d.npencil + C::B - 1) / C::B), (unsigned)ncomp);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_fast_c2c<T, N, -1>), grid, dim3(C::NT), 0, s, in, out, d, tw, scale);
+  else
+    hipLaunchKernelGGL((k_fast_c2c<T, N, 1>), grid, dim3(C::NT), 0, s, in, out, d, tw, scale);
+}
+
+template <typename T>
+static bool try_fast_c2c(int n, int dir, const cpx<T>* in, cpx<T>* out, const Strided& d, int ncomp,
+                         const cpx<T>* tw, T scale, hipStream_t s) {
+  switch (n) {
+    case 32: launch_fast_c2c<T, 32>(dir, in, out, d, ncomp, tw, scale, s); return true;
+    case 64: launch_fast_c2c<T, 64>(dir, in, out, d, ncomp, tw, scale, s); return true;
+    case 128: launch_fast_c2c<T, 128>(dir, in, out, d, ncomp, tw, scale, s); return true;
+    case 256: launch_fast_c2c<T, 256>(dir, in, out, d, ncomp, tw, scale, s); return true;
+    case 512: launch_fast_c2c<T, 512>(dir, in, out, d, ncomp, tw, scale, s); return true;
+    case 1024: launch_fast_c2c<T, 1024>(dir, in, out, d, ncomp, tw, scale, s); return true;
+  }
+  return false;
+}
+
+template <typename T, int N>
+static void launch_fast_x(int dir, const void* in, void* out, long long nrows, RowG g,
+                          const cpx<T>* tw, T scale, hipStream_t s) {
+  using C = Cfg<T, N>;
+  const long long pairs = (nrows + 1) / 2;
+  dim3 grid((unsigned)((pairs + C::U - 1) / C::U));
+  if (dir < 0)
+    hipLaunchKernelGGL((k_fast_r2c_x<T, N>), grid, dim3(C::XNT), 0, s, (const T*)in, (cpx<T>*)out,
+                       nrows, g, tw);
+  else
+    hipLaunchKernelGGL((k_fast_c2r_x<T, N>), grid, dim3(C::XNT), 0, s, (const cpx<T>*)in, (T*)out,
+                       nrows, g, tw, scale);
+}
+
+template <typename T>
+static bool try_fast_x(int n, int dir, const void* in, void* out, long long nrows, RowG g,
+                       const cpx<T>* tw, T scale, hipStream_t s) {
+  switch (n) {
+    case 32: launch_fast_x<T, 32>(dir, in, out, nrows, g, tw, scale, s); return true;
+    case 64: launch_fast_x<T, 64>(dir, in, out, nrows, g, tw, scale, s); return true;
+    case 128: launch_fast_x<T, 128>(dir, in, out, nrows, g, tw, scale, s); return true;
+    case 256: launch_fast_x<T, 256>(dir, in, out, nrows, g, tw, scale, s); return true;
+    case 512: launch_fast_x<T, 512>(dir, in, out, nrows, g, tw, scale, s); return true;
+    case 1024: launch_fast_x<T, 1024>(dir, in, out, nrows, g, tw, scale, s); return true;
+  }
+  return false;
+}
+
+// ----------------------------------------------------------------------------
 // host launchers
 // ----------------------------------------------------------------------------
 static int ilog2(int b) {
@@ -142,8 +284,10 @@ static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* i
   d.B = pick_b(ap.n, Lds<T>::kCap);
   d.bshift = ilog2(d.B);
   d.ld = ap.n + 1;
-  dim3 grid((unsigned)((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(p->tw[axis]);
+  if (!p->generic_only && try_fast_c2c<T>(ap.n, dir, in, out, d, ncomp, tw, scale, s))
+    return check_launch("c2c pass");
+  dim3 grid((unsigned)((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
   if (dir < 0)
     hipLaunchKernelGGL((k_c2c_strided<T, -1>), grid, dim3(kNT), 0, s, in, out, d, ap, tw, scale);
   else
@@ -155,11 +299,15 @@ template <typename T>
 static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, hipStream_t s) {
   const int nx = p->n[0], nh = p->nh;
   const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
-  const int B = pick_b(nx, Lds<T>::kCap);
-  const long long pairs = (nrows + 1) / 2;
-  dim3 grid((unsigned)((pairs + B - 1) / B));
-  hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
-                     p->ax[0], reinterpret_cast<const cpx<T>*>(p->tw[0]));
+  const cpx<T>* twx = reinterpret_cast<const cpx<T>*>(p->tw[0]);
+  const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
+  if (p->generic_only || !try_fast_x<T>(nx, -1, phys, spec, nrows, rg, twx, T(1), s)) {
+    const int B = pick_b(nx, Lds<T>::kCap);
+    const long long pairs = (nrows + 1) / 2;
+    dim3 grid((unsigned)((pairs + B - 1) / B));
+    hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
+                       p->ax[0], twx);
+  }
   gbg_status st = check_launch("r2c x pass");
   if (st) return st;
   if ((st = launch_strided<T>(p, 1, -1, spec, spec, ncomp, T(1), s))) return st;
@@ -174,12 +322,16 @@ static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, 
   if ((st = launch_strided<T>(p, 1, 1, work, work, ncomp, T(1), s))) return st;
   const int nx = p->n[0], nh = p->nh;
   const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
-  const int B = pick_b(nx, Lds<T>::kCap);
-  const long long pairs = (nrows + 1) / 2;
-  dim3 grid((unsigned)((pairs + B - 1) / B));
   const T scale = T(1.0 / ((double)p->n[0] * (double)p->n[1] * (double)p->n[2]));
-  hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,
-                     p->ax[0], reinterpret_cast<const cpx<T>*>(p->tw[0]), scale);
+  const cpx<T>* twx = reinterpret_cast<const cpx<T>*>(p->tw[0]);
+  const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
+  if (p->generic_only || !try_fast_x<T>(nx, 1, work, phys, nrows, rg, twx, scale, s)) {
+    const int B = pick_b(nx, Lds<T>::kCap);
+    const long long pairs = (nrows + 1) / 2;
+    dim3 grid((unsigned)((pairs + B - 1) / B));
+    hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,
+                       p->ax[0], twx, scale);
+  }
   return check_launch("c2r x pass");
 }
 

@@ -63,6 +63,17 @@ struct AxisPlan {
 // 13).  Returns false when n has a larger prime factor.
 bool factor_axis(int n, AxisPlan* ap);
 
+// FP64 per-axis wavenumber tables (device pointers): derivative family kd*,
+// squared family ks*.
+struct Tables {
+  const double* kd1;
+  const double* kd2;
+  const double* kd3;
+  const double* ks1;
+  const double* ks2;
+  const double* ks3;
+};
+
 // ---------------------------------------------------------------------------
 // The plan object behind gbg_plan*
 // ---------------------------------------------------------------------------
@@ -79,6 +90,7 @@ struct gbg_plan {
   size_t M;          // stored modes per component
   size_t P;          // grid points per component
   int device;
+  int generic_only;  // GBG_GENERIC_FFT=1 at creation: mixed-radix LDS passes only
   gbg::AxisPlan ax[3];
   // FP64 per-axis tables (device): derivative family and squared family
   double* kd[3];
@@ -111,6 +123,10 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 // ensure the plan's work area holds at least `bytes`
 gbg_status ensure_work(gbg_plan* p, size_t bytes);
+
+inline Tables tables_of(gbg_plan* p) {
+  return Tables{p->kd[0], p->kd[1], p->kd[2], p->ks[0], p->ks[1], p->ks[2]};
+}
 
 // event marks for the profiling aid
 void prof_begin(gbg_plan* p, hipStream_t s);

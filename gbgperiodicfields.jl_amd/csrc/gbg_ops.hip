@@ -23,19 +23,6 @@ struct ModeIdx {
   }
 };
 
-struct Tables {
-  const double* kd1;
-  const double* kd2;
-  const double* kd3;
-  const double* ks1;
-  const double* ks2;
-  const double* ks3;
-};
-
-static Tables tables_of(gbg_plan* p) {
-  return Tables{p->kd[0], p->kd[1], p->kd[2], p->ks[0], p->ks[1], p->ks[2]};
-}
-
 // spectral_band membership in FP64: lo2 <= (k1^2 + k2^2) + k3^2 <= hi2
 __device__ __forceinline__ bool in_band(const Tables& t, unsigned kx, unsigned ky, unsigned kz,
                                         double lo2, double hi2) {

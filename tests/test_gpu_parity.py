@@ -166,17 +166,22 @@ def test_operators(G, n, L, dtype):
     assert abs(ip - ref) <= 1e-12 * abs(ref) + 1e-300
 
 
+UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), ((48, 32, 40), None),
+             ((64, 64, 128), None), ((128, 128, 32), None), ((64, 64, 64), 5.0), ((64, 64, 64), 1000.0),
+             ((32, 32, 64), 7.3)]
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("n", [(16, 16, 16), (64, 64, 64), (48, 32, 40)])
+@pytest.mark.parametrize("n,kc", UXW_CASES)
 @pytest.mark.parametrize("fused", [False, True])
-def test_uxw_parity(G, n, dtype, fused):
-    say(f"[uxw] n={n} dtype={np.dtype(dtype).name} fused={fused}")
+def test_uxw_parity(G, n, kc, dtype, fused):
+    say(f"[uxw] n={n} kc={kc} dtype={np.dtype(dtype).name} fused={fused}")
     g = ogrid(n)
     m = O.Multipliers.build(g)
     grid = G.Grid3D(n)
     plans = G.FieldTransformPlans(grid, dtype)
     u = O.synthetic_field(g, 2001, vector=True, m=m)
-    kc = O.dealias_cutoff(n[0])
+    kc = O.dealias_cutoff(n[0]) if kc is None else kc
     ref, E, Z = O.uxw(u, g, kc, m)
     du = G.Field(grid, "spectral", 1, dtype).upload(u)
     dn = G.Field(grid, "spectral", 1, dtype)
