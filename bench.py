@@ -129,6 +129,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES device-to-device twice (a plain copy kernel of known "
+                         "size, the scale for the HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -147,6 +150,19 @@ def main():
     ev = HipEvents()
     stream = G.Stream()
     print(f"[bench] {n} {args.dtype} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration
+        src = C.c_void_p()
+        dst = C.c_void_p()
+        G.check(G.load().gbg_malloc(C.byref(src), nb))
+        G.check(G.load().gbg_malloc(C.byref(dst), nb))
+        G.check(G.load().gbg_memset(src.value, 1, nb, stream.handle))
+        for _ in range(2):
+            G.check(G.load().gbg_memcpy_d2d(dst.value, src.value, nb, stream.handle))
+        stream.synchronize()
+        G.check(G.load().gbg_free(src.value))
+        G.check(G.load().gbg_free(dst.value))
+        print(f"[bench] copy calibration: 2 x {nb} bytes device-to-device", file=sys.stderr, flush=True)
 
     r = run_gpu(G, n, dtype, args.steps, args.warmup, stream, ev)
     ms = r["total_ms"] / args.steps

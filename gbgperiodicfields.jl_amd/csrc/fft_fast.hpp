@@ -134,6 +134,17 @@ struct Ld {
   static constexpr int value = N + (sizeof(T) == 8 ? 2 : 4);
 };
 
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Giving each XCD a contiguous run of logical tiles makes neighbouring tiles,
+// which share cache lines at their edges, meet in the same L2.  The grid is
+// launched with xcd_grid(nb) >= nb workgroups; logical ids >= nb are idle.
+constexpr int kXcd = 8;
+inline unsigned xcd_grid(long long nb) { return (unsigned)(((nb + kXcd - 1) / kXcd) * kXcd); }
+__device__ __forceinline__ long long xcd_tile() {
+  const long long per = gridDim.x / kXcd;
+  return (long long)(blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
+
 // per-length launch shape: E elements per thread, TP threads per pencil,
 // B pencils per workgroup of the strided passes
 template <typename T, int N>

@@ -20,6 +20,7 @@
 // Buffers: W1 = [6][Nz][nyc][nhk] (K1 -> K2), reused as [3][Nz][nyc][nhk]
 // (K4 -> K5); W2 = [6][Nz][Ny][nhk] (K2 -> K3), reused as [3][Nz][Ny][nhk]
 // (K3 -> K4).  nhk = kx_max + 1, nyc = number of kept ky indices.
+#include <type_traits>
 #include <vector>
 
 #include "fft_fast.hpp"
@@ -67,12 +68,13 @@ gbg_status pipe_uxw_composed(gbg_plan* p, const void* u_hat, void* n_hat, double
 struct Geo {
   int nx, ny, nz, nh;
   int nhk;          // kept kx indices [0, nhk)
+  int nhp;          // row pitch of the compact buffers (nhk rounded up to 16)
   int nyc;          // kept ky indices, compact
   int ya;           // kyc < ya -> ky = kyc; else ky = ny - nyc + kyc
   int nyq_x;        // x Nyquist index (nh - 1) for even nx, else -1
   long long M;      // full spectral component
-  long long s1;     // W1 component stride  nz * nyc * nhk
-  long long s2;     // W2 component stride  nz * ny * nhk
+  long long s1;     // compact-ky component stride  nz * nyc * nhp
+  long long s2;     // full-y component stride      nz * ny * nhp
   double hi2;       // kc^2 (FP64)
 };
 
@@ -89,26 +91,33 @@ __device__ __forceinline__ int kyc_of(int ky, const Geo& g) {
 }
 
 // ---- K1: z-inverse of u^ and curl u^ on the disk pencils --------------------
+// Tiles: B consecutive kx of one compact (kyc) row, pencil index
+// p1 = kyc * nhp + kx.  Energy and enstrophy partials land in part[2 b],
+// part[2 b + 1] for the launched workgroup b (zeros for idle ones).
 template <typename T, int N>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ w1, Geo g, Tables tb,
-    const cpx<T>* __restrict__ tw, double* __restrict__ part) {
+    const cpx<T>* __restrict__ tw, double* __restrict__ part, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[B * LD];
   __shared__ double red[2][C::NT / 64];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const int np = g.nyc * g.nhk;
-  const int p1 = blockIdx.x * B + pl;
+  const long long tile = fast::xcd_tile();
+  const long long p1 = tile * B + pl;
   int kx = 0, ky = 0;
   double qxy = 0.0;
   bool disk = false;
-  if (p1 < np) {
-    const int kyc = p1 / g.nhk;
-    kx = p1 - kyc * g.nhk;
-    ky = ky_of(kyc, g);
-    qxy = tb.ks1[kx] + tb.ks2[ky];
-    disk = qxy <= g.hi2;
+  if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
+    const int kyc = (int)(p1 / g.nhp);
+    kx = (int)(p1 - (long long)kyc * g.nhp);
+    if (kx < g.nhk) {
+      ky = ky_of(kyc, g);
+      qxy = tb.ks1[kx] + tb.ks2[ky];
+      disk = qxy <= g.hi2;
+    } else {
+      kx = 0;
+    }
   }
   if (!__syncthreads_or(disk)) {
     if (threadIdx.x == 0) {
@@ -126,7 +135,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   for (int m = 0; m < E; ++m) keep[m] = disk && (qxy + tb.ks3[t + TP * m] <= g.hi2);
   double eacc = 0.0, zacc = 0.0;
   cpx<T>* pen = lds + pl * LD;
-  const long long zs = (long long)g.nyc * g.nhk;
+  const long long zs = (long long)g.nyc * g.nhp;
   // Each output field reloads the components it needs (the input lives on
   // the 2/3 ball only, so the repeat reads are few and mostly cache hits):
   //   f = 0,1,2: u_f;   f = 3,4,5: w = i (ca u_ia - cb u_ib) with
@@ -207,41 +216,46 @@ __global__ __launch_bounds__(256) void k_uxw_reduce(const double* __restrict__ p
   }
 }
 
-// ---- K2 / K4: y passes over (z, kx <= kx_max) pencils --------------------------
-// DIR = +1 (inverse, 6 fields): loads the disk ky from W1, stores all y to W2.
-// DIR = -1 (forward, 3 fields): loads all y from W2, stores the disk ky to W1.
+// ---- K2 / K4: y passes over (z, kx < nhk) pencils ------------------------------
+// DIR = +1 (inverse, 6 fields): loads the disk ky from the compact buffer,
+//   stores all y to the full-y buffer.
+// DIR = -1 (forward, 3 fields): loads all y from the full-y buffer, stores
+//   the disk ky to the compact buffer.
+// The next field's loads are issued before the current field's transform.
 template <typename T, int N, int DIR>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restrict__ src,
                                                          cpx<T>* __restrict__ dst, Geo g,
-                                                         Tables tb, const cpx<T>* __restrict__ tw) {
+                                                         Tables tb, const cpx<T>* __restrict__ tw,
+                                                         long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   constexpr int NF = DIR > 0 ? 6 : 3;
   __shared__ cpx<T> lds[B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const int np = g.nz * g.nhk;
-  const int p2 = blockIdx.x * B + pl;
-  const bool valid = p2 < np;
+  const long long tile = fast::xcd_tile();
+  const long long p2 = tile * B + pl;
   int z = 0, kx = 0;
-  if (valid) {
-    z = p2 / g.nhk;
-    kx = p2 - z * g.nhk;
+  bool valid = false;
+  if (tile < ntiles && p2 < (long long)g.nz * g.nhp) {
+    z = (int)(p2 / g.nhp);
+    kx = (int)(p2 - (long long)z * g.nhp);
+    valid = kx < g.nhk;
+    if (!valid) kx = 0;
   }
   const double q1 = tb.ks1[kx];
-  // compact (disk) offsets of this thread's elements; -1 = not stored
+  // compact (disk) offsets of this thread's elements; -1 = not held
   int coff[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int ky = t + TP * m;
     const int kyc = kyc_of(ky, g);
-    coff[m] = (valid && kyc >= 0 && q1 + tb.ks2[ky] <= g.hi2) ? kyc * g.nhk : -1;
+    coff[m] = (valid && kyc >= 0 && q1 + tb.ks2[ky] <= g.hi2) ? kyc * g.nhp : -1;
   }
-  const long long cbase = (long long)z * g.nyc * g.nhk + kx;   // W1 (compact ky)
-  const long long fbase = (long long)z * g.ny * g.nhk + kx;    // W2 (all y)
+  const long long cbase = (long long)z * g.nyc * g.nhp + kx;   // compact ky
+  const long long fbase = (long long)z * g.ny * g.nhp + kx;    // all y
   cpx<T>* pen = lds + pl * LD;
-#pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
-    cpx<T> v[E];
+  cpx<T> nxt[E];
+  auto load = [&](int f, cpx<T>(&v)[E]) {
     if (DIR > 0) {
       const cpx<T>* in = src + f * g.s1 + cbase;
 #pragma unroll
@@ -250,14 +264,22 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
       const cpx<T>* in = src + f * g.s2 + fbase;
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        v[m] = valid ? in[(long long)(t + TP * m) * g.nhk] : mk<T>(T(0), T(0));
+        v[m] = valid ? in[(long long)(t + TP * m) * g.nhp] : mk<T>(T(0), T(0));
     }
+  };
+  load(0, nxt);
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = nxt[m];
+    if (f + 1 < NF) load(f + 1, nxt);
     fast::fft<T, N, E, DIR, 0>(v, pen, t, tw);
     if (DIR > 0) {
       if (valid) {
         cpx<T>* o = dst + f * g.s2 + fbase;
 #pragma unroll
-        for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * g.nhk] = v[m];
+        for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * g.nhp] = v[m];
       }
     } else {
       cpx<T>* o = dst + f * g.s1 + cbase;
@@ -270,31 +292,33 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 
 // ---- K3: x pass -- C2R of 6 rows, u x w, R2C of 3 rows ------------------------
 // One unit = a pair of rows (r0, r1) of the flattened (y, z) index; the two
-// rows' n3 share one complex transform.
+// rows' n3 share one complex transform.  Rows have pitch nhp.
 template <typename T, int N>
-__global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_uxw_x(const cpx<T>* __restrict__ w2,
-                                                          cpx<T>* __restrict__ w3, Geo g,
-                                                          const cpx<T>* __restrict__ tw, T scale) {
+__global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
+                                                             cpx<T>* __restrict__ w3, Geo g,
+                                                             const cpx<T>* __restrict__ tw, T scale,
+                                                             long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, LD = C::LD, S = C::XSYNC;
   __shared__ cpx<T> lds[C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
+  const long long tile = fast::xcd_tile();
   const long long nrows = (long long)g.ny * g.nz;
-  const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
-  const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
+  const long long r0 = 2 * (tile * C::U + un), r1 = r0 + 1;
+  const bool ok0 = tile < ntiles && r0 < nrows, ok1 = tile < ntiles && r1 < nrows;
   cpx<T>* pen = lds + un * LD;
   T n3a[E], n3b[E];
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     const long long r = h ? r1 : r0;
     const bool ok = h ? ok1 : ok0;
-    const long long ro = ok ? r * g.nhk : 0;
+    const long long ro = ok ? r * g.nhp : 0;
     T ph[6][E];
 #pragma unroll
     for (int pi = 0; pi < 3; ++pi) {
       cpx<T> v[E];
-      fast::c2r_pair_load<T, N, E>(v, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok, ok,
-                                    t, g);
+      fast::c2r_pair_load<T, N, E>(v, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok,
+                                   ok, t, g);
       fast::fft<T, N, E, 1, S>(v, pen, t, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -322,22 +346,25 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_uxw_x(const cpx<T>* __rest
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = mk<T>(n3a[m], n3b[m]);
   fast::fft<T, N, E, -1, S>(v, pen, t, tw);
-  fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhk : 0),
-                       w3 + 2 * g.s2 + (ok1 ? r1 * g.nhk : 0), ok0, ok1, t, g);
+  fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhp : 0),
+                                w3 + 2 * g.s2 + (ok1 ? r1 * g.nhp : 0), ok0, ok1, t, g);
 }
 
 // ---- K5: z-forward, Leray projection, mask, mean-free ------------------------
+// Tiles: B consecutive pencils of the output's flattened (ky, kx) index, so
+// the (largest) output stream is written in aligned segments.
 template <typename T, int N>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ w1, cpx<T>* __restrict__ nh, Geo g, Tables tb,
-    const cpx<T>* __restrict__ tw) {
+    const cpx<T>* __restrict__ tw, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const int np = g.ny * g.nh;
-  const int p5 = blockIdx.x * B + pl;
-  const bool valid = p5 < np;
+  const long long tile = fast::xcd_tile();
+  const long long p5l = tile * B + pl;
+  const bool valid = tile < ntiles && p5l < (long long)g.ny * g.nh;
+  const int p5 = valid ? (int)p5l : 0;
   int kx = 0, ky = 0, kyc = -1;
   double qxy = 0.0;
   bool disk = false;
@@ -358,8 +385,8 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     }
     return;
   }
-  const long long zs = (long long)g.nyc * g.nhk;
-  const long long cb = (long long)(disk ? kyc : 0) * g.nhk + kx;
+  const long long zs = (long long)g.nyc * g.nhp;
+  const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
   cpx<T>* pen = lds + pl * LD;
   cpx<T> n[3][E];
 #pragma unroll
@@ -397,6 +424,20 @@ static bool fast_len(int n) {
   return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
 }
 
+// call f(std::integral_constant<int, n>) for a supported power-of-two length
+template <typename F>
+static bool with_len(int n, F&& f) {
+  switch (n) {
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    case 512: f(std::integral_constant<int, 512>{}); return true;
+    case 1024: f(std::integral_constant<int, 1024>{}); return true;
+  }
+  return false;
+}
+
 static void axis_host(int n, double L, bool half, std::vector<double>& ks) {
   const double unit = (2.0 * M_PI) / L;
   const int stored = half ? n / 2 + 1 : n;
@@ -427,26 +468,24 @@ static Geo make_geo(gbg_plan* p, double kc) {
   int b = 0;
   while (b < g.ny - (g.ny + 1) / 2 && s2[g.ny - 1 - b] <= g.hi2) ++b;
   g.nhk = nhk < 1 ? 1 : nhk;
+  g.nhp = (g.nhk + 15) / 16 * 16;
   g.ya = ya;
   g.nyc = ya + b;
-  g.s1 = (long long)g.nz * g.nyc * g.nhk;
-  g.s2 = (long long)g.nz * g.ny * g.nhk;
+  g.s1 = (long long)g.nz * g.nyc * g.nhp;
+  g.s2 = (long long)g.nz * g.ny * g.nhp;
   return g;
 }
 
-template <typename T, int NX, int NY, int NZ>
-static gbg_status uxw_fused_impl(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
-                                 hipStream_t s) {
+template <typename T>
+static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
+                            hipStream_t s) {
   const size_t cb = sizeof(cpx<T>);
-  // W1 holds K1's 6 compact fields, later K3's 3 full-y fields
+  // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
+  // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields
   const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cb;
   const size_t b2 = (size_t)6 * g.s2 * cb;
-  using CZ = Cfg<T, NZ>;
-  using CY = Cfg<T, NY>;
-  using CX = Cfg<T, NX>;
-  const unsigned nb1 = (unsigned)(((long long)g.nyc * g.nhk + CZ::B - 1) / CZ::B);
-  const size_t pbytes = sizeof(double) * 2 * nb1;
-  gbg_status st = ensure_work(p, b1 + b2 + pbytes);
+  const long long part_max = (long long)g.nyc * g.nhp + fast::kXcd;   // >= K1 grid
+  gbg_status st = ensure_work(p, b1 + b2 + sizeof(double) * 2 * part_max);
   if (st) return st;
   char* base = (char*)p->work;
   cpx<T>* W1 = (cpx<T>*)base;
@@ -456,75 +495,67 @@ static gbg_status uxw_fused_impl(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, con
   const cpx<T>* twx = (const cpx<T>*)p->tw[0];
   const cpx<T>* twy = (const cpx<T>*)p->tw[1];
   const cpx<T>* twz = (const cpx<T>*)p->tw[2];
-  prof_begin(p, s);
-  hipLaunchKernelGGL((k_uxw_z_inverse<T, NZ>), dim3(nb1), dim3(CZ::NT), 0, s, uh, W1, g, tb, twz, part);
-  if ((st = check_launch("uxw K1"))) return st;
   const double P = (double)g.nx * g.ny * g.nz;
-  hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)nb1, P * P, p->red_out);
-  if ((st = check_launch("uxw reduce"))) return st;
+  prof_begin(p, s);
+  with_len(g.nz, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
+    const unsigned grid = fast::xcd_grid(nt);
+    hipLaunchKernelGGL((k_uxw_z_inverse<T, N>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g, tb, twz,
+                       part, nt);
+    hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
+  });
+  if ((st = check_launch("uxw K1"))) return st;
   prof_mark(p, s);
-  const unsigned nb2 = (unsigned)(((long long)g.nz * g.nhk + CY::B - 1) / CY::B);
-  hipLaunchKernelGGL((k_uxw_y<T, NY, 1>), dim3(nb2), dim3(CY::NT), 0, s, W1, W2, g, tb, twy);
+  with_len(g.ny, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_uxw_y<T, N, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
+                       tb, twy, nt);
+  });
   if ((st = check_launch("uxw K2"))) return st;
   prof_mark(p, s);
-  const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
-  const unsigned nb3 = (unsigned)((pairs + CX::U - 1) / CX::U);
-  const T scale = T(1.0 / P);
-  hipLaunchKernelGGL((k_uxw_x<T, NX>), dim3(nb3), dim3(CX::XNT), 0, s, W2, W1, g, twx, scale);
+  with_len(g.nx, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
+    const long long nt = (pairs + C::U - 1) / C::U;
+    hipLaunchKernelGGL((k_uxw_x<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2, W1, g,
+                       twx, T(1.0 / P), nt);
+  });
   if ((st = check_launch("uxw K3"))) return st;
   prof_mark(p, s);
-  // K3 wrote [3][Nz][Ny][nhk] into W1's memory; K4 reads it and writes the
-  // compact [3][Nz][nyc][nhk] into W2's memory.
-  hipLaunchKernelGGL((k_uxw_y<T, NY, -1>), dim3(nb2), dim3(CY::NT), 0, s, W1, W2, g, tb, twy);
+  with_len(g.ny, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_uxw_y<T, N, -1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
+                       tb, twy, nt);
+  });
   if ((st = check_launch("uxw K4"))) return st;
   prof_mark(p, s);
-  const unsigned nb5 = (unsigned)(((long long)g.ny * g.nh + CZ::B - 1) / CZ::B);
-  hipLaunchKernelGGL((k_uxw_z_forward<T, NZ>), dim3(nb5), dim3(CZ::NT), 0, s, W2, out, g, tb, twz);
+  with_len(g.nz, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.ny * g.nh + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2, out,
+                       g, tb, twz, nt);
+  });
   if ((st = check_launch("uxw K5"))) return st;
   prof_mark(p, s);
   return GBG_OK;
 }
 
-template <typename T, int NX, int NY>
-static gbg_status dispatch_z(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g, hipStream_t s) {
-  switch (g.nz) {
-    case 32: return uxw_fused_impl<T, NX, NY, 32>(p, uh, out, g, s);
-    case 64: return uxw_fused_impl<T, NX, NY, 64>(p, uh, out, g, s);
-    case 128: return uxw_fused_impl<T, NX, NY, 128>(p, uh, out, g, s);
-    case 256: return uxw_fused_impl<T, NX, NY, 256>(p, uh, out, g, s);
-    case 512: return uxw_fused_impl<T, NX, NY, 512>(p, uh, out, g, s);
-    case 1024: return uxw_fused_impl<T, NX, NY, 1024>(p, uh, out, g, s);
-  }
-  return GBG_E_UNSUPPORTED;
-}
-
-template <typename T>
-static gbg_status dispatch_fused(gbg_plan* p, const void* uh, void* out, const Geo& g, hipStream_t s) {
-  const cpx<T>* u = (const cpx<T>*)uh;
-  cpx<T>* o = (cpx<T>*)out;
-  // cubic grids and the anisotropic power-of-two boxes share instantiations
-  // per axis only when equal lengths repeat; keep the set small
-  if (g.nx == g.ny) {
-    switch (g.nx) {
-      case 32: return dispatch_z<T, 32, 32>(p, u, o, g, s);
-      case 64: return dispatch_z<T, 64, 64>(p, u, o, g, s);
-      case 128: return dispatch_z<T, 128, 128>(p, u, o, g, s);
-      case 256: return dispatch_z<T, 256, 256>(p, u, o, g, s);
-      case 512: return dispatch_z<T, 512, 512>(p, u, o, g, s);
-      case 1024: return dispatch_z<T, 1024, 1024>(p, u, o, g, s);
-    }
-  }
-  return GBG_E_UNSUPPORTED;
-}
-
 gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
                     hipStream_t s) {
-  const bool fusable = !(flags & GBG_UXW_UNFUSED) && fast_len(p->n[0]) && fast_len(p->n[1]) &&
-                       fast_len(p->n[2]) && p->n[0] == p->n[1];
+  const bool fusable = !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
+                       fast_len(p->n[1]) && fast_len(p->n[2]);
   if (!fusable) return pipe_uxw_composed(p, u_hat, n_hat, kc, s);
   const Geo g = make_geo(p, kc);
-  if (p->elem == 8) return dispatch_fused<double>(p, u_hat, n_hat, g, s);
-  return dispatch_fused<float>(p, u_hat, n_hat, g, s);
+  if (p->elem == 8) return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, s);
+  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, s);
 }
 
 gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,

@@ -72,6 +72,29 @@ def test_transforms(G, n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(64, 64, 64), (32, 128, 256), (512, 32, 32)])
+def test_transforms_generic_vs_fast(G, n, dtype, monkeypatch):
+    """Power-of-two lengths take the register-resident passes; the mixed-radix
+    LDS passes (GBG_GENERIC_FFT=1 at plan creation) must agree with them and
+    with the oracle."""
+    say(f"[generic-vs-fast] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    grid = G.Grid3D(n)
+    fast = G.FieldTransformPlans(grid, dtype)
+    monkeypatch.setenv("GBG_GENERIC_FFT", "1")
+    generic = G.FieldTransformPlans(grid, dtype)
+    monkeypatch.delenv("GBG_GENERIC_FFT")
+    f = rand_phys(g, 1, 21)
+    ref = O.to_spectral(f)
+    phys = G.Field(grid, "physical", 1, dtype).upload(f)
+    for plans in (fast, generic):
+        spec = G.to_spectral(phys, plans)
+        assert O.rel_l2(spec.download(), ref) < TOL[dtype]
+        back = G.to_physical(spec, plans)
+        assert O.rel_l2(back.download(), f) < 10 * TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n,L", [((16, 12, 10), (O.TWO_PI,) * 3), ((8, 7, 6), (1.0, 2.5, 7.0)),
                                  ((64, 64, 64), (O.TWO_PI,) * 3)])
 def test_operators(G, n, L, dtype):
