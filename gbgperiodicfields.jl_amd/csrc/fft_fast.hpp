@@ -69,8 +69,21 @@ __device__ __forceinline__ void twiddle(cpx<T>* a, int k, int step, const cpx<T>
   }
 }
 
-template <typename T, int N, int E, int LS, int DIR, int SYNC>
-__device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* pen, int t,
+constexpr int num_stages(int n) {
+  int s = 0, ls = 1;
+  while (ls < n) {
+    const int rem = n / ls;
+    ls *= (rem % 8 == 0) ? 8 : rem;
+    ++s;
+  }
+  return s;
+}
+
+// Stage (R, LS) of the ping-pong schedule: exchange k (k = 0, 1, ...) writes
+// buffer k mod 2 and the next stage reads it, so one barrier per exchange
+// suffices -- a buffer is rewritten only two exchanges after it was read.
+template <typename T, int N, int E, int LS, int DIR, int SYNC, int W>
+__device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                        const cpx<T>* __restrict__ tw) {
   constexpr int R = StageR<N, LS>::R;
   static_assert(E % R == 0, "elements per thread must be a multiple of each radix");
@@ -79,6 +92,8 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* pen, int t,
   constexpr int TP = N / E;
   constexpr bool FIRST = (LS == 1);
   constexpr bool LAST = (LS * R == N);
+  cpx<T>* const rd = W == 0 ? p1 : p0;   // written by the previous stage
+  cpx<T>* const pen = W == 0 ? p0 : p1;  // written by this stage
   cpx<T> a[NB][R];
   if constexpr (FIRST) {
 #pragma unroll
@@ -90,9 +105,8 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* pen, int t,
     for (int b = 0; b < NB; ++b) {
       const int j = t + TP * b;
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[b][r] = pen[j + r * NR];
+      for (int r = 0; r < R; ++r) a[b][r] = rd[j + r * NR];
     }
-    lds_sync<SYNC>();
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -117,15 +131,19 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* pen, int t,
       for (int r = 0; r < R; ++r) o[r * LS] = a[b][r];
     }
     lds_sync<SYNC>();
-    stages<T, N, E, LS * R, DIR, SYNC>(v, pen, t, tw);
+    stages<T, N, E, LS * R, DIR, SYNC, 1 - W>(v, p0, p1, t, tw);
   }
 }
 
-// In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  `pen` is
-// this pencil's LDS region (N elements); it is free again on return.
+// In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  p0, p1 are
+// this pencil's two LDS regions (N elements each).  On return p0 may be
+// written at once (its last read was followed by a barrier); p1 may not.
 template <typename T, int N, int E, int DIR, int SYNC>
-__device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* pen, int t, const cpx<T>* __restrict__ tw) {
-  stages<T, N, E, 1, DIR, SYNC>(v, pen, t, tw);
+__device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
+                                    const cpx<T>* __restrict__ tw) {
+  stages<T, N, E, 1, DIR, SYNC, 0>(v, p0, p1, t, tw);
+  // with an odd number of exchanges the last one read p0
+  if constexpr ((num_stages(N) - 1) % 2 == 1) lds_sync<SYNC>();
 }
 
 // LDS pencil stride: padding spreads the pencils of a tile over the banks.

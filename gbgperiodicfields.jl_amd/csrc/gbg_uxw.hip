@@ -100,7 +100,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ tw, double* __restrict__ part, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  __shared__ cpx<T> lds[B * LD];
+  __shared__ cpx<T> lds[2 * B * LD];
   __shared__ double red[2][C::NT / 64];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
@@ -135,35 +135,47 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   for (int m = 0; m < E; ++m) keep[m] = disk && (qxy + tb.ks3[t + TP * m] <= g.hi2);
   double eacc = 0.0, zacc = 0.0;
   cpx<T>* pen = lds + pl * LD;
+  cpx<T>* pen1 = lds + (B + pl) * LD;
   const long long zs = (long long)g.nyc * g.nhp;
   // Each output field reloads the components it needs (the input lives on
   // the 2/3 ball only, so the repeat reads are few and mostly cache hits):
   //   f = 0,1,2: u_f;   f = 3,4,5: w = i (ca u_ia - cb u_ib) with
   //   w1 = i(a2 u3 - a3 u2), w2 = i(a3 u1 - a1 u3), w3 = i(a1 u2 - a2 u1)
-#pragma unroll 1
-  for (int f = 0; f < 6; ++f) {
+  // The loads of field f + 1 are issued before the transform of field f.
+  cpx<T> nA[E], nB[E];
+  auto load = [&](int f) {
     const int ia = f < 3 ? f : (f == 3 ? 2 : (f == 4 ? 0 : 1));
     const int ib = f == 3 ? 1 : (f == 4 ? 2 : 0);
     const cpx<T>* ua = uh + ia * g.M + off;
     const cpx<T>* ub = uh + ib * g.M + off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const long long zo = (long long)(t + TP * m) * plane;
+      nA[m] = keep[m] ? ua[zo] : mk<T>(T(0), T(0));
+      if (f >= 3) nB[m] = keep[m] ? ub[zo] : mk<T>(T(0), T(0));
+    }
+  };
+  load(0);
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
     cpx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int z = t + TP * m;
-      const cpx<T> A = keep[m] ? ua[z * plane] : mk<T>(T(0), T(0));
+      const cpx<T> A = nA[m];
       if (f < 3) {
         v[m] = A;
         eacc += wx * ((double)A.x * A.x + (double)A.y * A.y);
       } else {
-        const cpx<T> Bv = keep[m] ? ub[z * plane] : mk<T>(T(0), T(0));
-        const T a3 = T(tb.kd3[z]);
+        const cpx<T> Bv = nB[m];
+        const T a3 = T(tb.kd3[t + TP * m]);
         const T ca = f == 3 ? a2 : (f == 4 ? a3 : a1);
         const T cb = f == 3 ? a3 : (f == 4 ? a1 : a2);
         v[m] = cmuli(csub(cscale(A, ca), cscale(Bv, cb)));
         zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
       }
     }
-    fast::fft<T, N, E, 1, 0>(v, pen, t, tw);
+    if (f + 1 < 6) load(f + 1);
+    fast::fft<T, N, E, 1, 0>(v, pen, pen1, t, tw);
     if (disk) {
       cpx<T>* o = w1 + f * g.s1 + p1;
 #pragma unroll
@@ -230,7 +242,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   constexpr int NF = DIR > 0 ? 6 : 3;
-  __shared__ cpx<T> lds[B * LD];
+  __shared__ cpx<T> lds[2 * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
   const long long p2 = tile * B + pl;
@@ -254,6 +266,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   const long long cbase = (long long)z * g.nyc * g.nhp + kx;   // compact ky
   const long long fbase = (long long)z * g.ny * g.nhp + kx;    // all y
   cpx<T>* pen = lds + pl * LD;
+  cpx<T>* pen1 = lds + (B + pl) * LD;
   cpx<T> nxt[E];
   auto load = [&](int f, cpx<T>(&v)[E]) {
     if (DIR > 0) {
@@ -274,7 +287,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = nxt[m];
     if (f + 1 < NF) load(f + 1, nxt);
-    fast::fft<T, N, E, DIR, 0>(v, pen, t, tw);
+    fast::fft<T, N, E, DIR, 0>(v, pen, pen1, t, tw);
     if (DIR > 0) {
       if (valid) {
         cpx<T>* o = dst + f * g.s2 + fbase;
@@ -300,13 +313,14 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
                                                              long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, LD = C::LD, S = C::XSYNC;
-  __shared__ cpx<T> lds[C::U * LD];
+  __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long tile = fast::xcd_tile();
   const long long nrows = (long long)g.ny * g.nz;
   const long long r0 = 2 * (tile * C::U + un), r1 = r0 + 1;
   const bool ok0 = tile < ntiles && r0 < nrows, ok1 = tile < ntiles && r1 < nrows;
   cpx<T>* pen = lds + un * LD;
+  cpx<T>* pen1 = lds + (C::U + un) * LD;
   T n3a[E], n3b[E];
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
@@ -316,14 +330,14 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
     T ph[6][E];
 #pragma unroll
     for (int pi = 0; pi < 3; ++pi) {
-      cpx<T> v[E];
-      fast::c2r_pair_load<T, N, E>(v, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok,
+      cpx<T> z[E];
+      fast::c2r_pair_load<T, N, E>(z, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok,
                                    ok, t, g);
-      fast::fft<T, N, E, 1, S>(v, pen, t, tw);
+      fast::fft<T, N, E, 1, S>(z, pen, pen1, t, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        ph[2 * pi][m] = v[m].x * scale;
-        ph[2 * pi + 1][m] = v[m].y * scale;
+        ph[2 * pi][m] = z[m].x * scale;
+        ph[2 * pi + 1][m] = z[m].y * scale;
       }
     }
     // (u x w)_i = u_j w_k - u_k w_j; ph = u1, u2, u3, w1, w2, w3
@@ -339,13 +353,13 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
         n3b[m] = n3v;
       v[m] = mk<T>(n1, n2);
     }
-    fast::fft<T, N, E, -1, S>(v, pen, t, tw);
+    fast::fft<T, N, E, -1, S>(v, pen, pen1, t, tw);
     fast::r2c_pair_store<T, N, E>(v, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
   }
   cpx<T> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = mk<T>(n3a[m], n3b[m]);
-  fast::fft<T, N, E, -1, S>(v, pen, t, tw);
+  fast::fft<T, N, E, -1, S>(v, pen, pen1, t, tw);
   fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhp : 0),
                                 w3 + 2 * g.s2 + (ok1 ? r1 * g.nhp : 0), ok0, ok1, t, g);
 }
@@ -359,7 +373,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ tw, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  __shared__ cpx<T> lds[B * LD];
+  __shared__ cpx<T> lds[2 * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
   const long long p5l = tile * B + pl;
@@ -388,14 +402,16 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   const long long zs = (long long)g.nyc * g.nhp;
   const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
   cpx<T>* pen = lds + pl * LD;
+  cpx<T>* pen1 = lds + (B + pl) * LD;
   cpx<T> n[3][E];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const cpx<T>* in = w1 + c * g.s1 + cb;
 #pragma unroll
     for (int m = 0; m < E; ++m) n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
-    fast::fft<T, N, E, -1, 0>(n[c], pen, t, tw);
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0>(n[c], pen, pen1, t, tw);
   if (!valid) return;
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
 #pragma unroll
