@@ -83,6 +83,25 @@ def stage_bytes(n, elem):
             "K5_z_forward_project": 6 * comp}
 
 
+KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}>", "K2_y_inverse": "k_uxw_y<{t}, {n}, 1>",
+                   "K3_x_product": "k_uxw_x<{t}, {n}>", "K4_y_forward": "k_uxw_y<{t}, {n}, -1>",
+                   "K5_z_forward_project": "k_uxw_z_forward<{t}, {n}>"}
+
+
+def measured_traffic(stage, n, dtype):
+    """HBM bytes per launch of a pass, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (copy-kernel calibrated), or None if none match."""
+    path = os.path.join(REPO, "profiles", f"uxw_{n[0]}_hbm_traffic.json")
+    if not os.path.exists(path) or len(set(n)) != 1:
+        return None
+    with open(path) as fh:
+        data = json.load(fh)
+    t = "double" if dtype == np.float64 else "float"
+    k = KERNEL_OF_STAGE[stage].format(t=t, n=n[0])
+    rec = data.get("kernels", {}).get(k)
+    return None if rec is None else int(rec["hbm_bytes"])
+
+
 def run_gpu(G, n, dtype, steps, warmup, stream, ev, seed=2001):
     grid = G.Grid3D(n)
     plans = G.FieldTransformPlans(grid, dtype)
@@ -182,8 +201,8 @@ def main():
         i = int(np.argmax(stages))
         ach = sb[names[i]] / (stages[i] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": names[i], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_ms": round(stages[i], 4)}
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(names[i], n, dtype), "avg_ms": round(stages[i], 4)}
     pipeline_gbs = B / (ms * 1e-3) / 1e9
     out = {
         "metric": "512^3 dealiased u x omega fields/s (FP64) + % HBM roofline; FP32 vs FP64; p99 latency",

@@ -258,6 +258,38 @@ def test_config1_pipeline(G, dtype):
     assert O.rel_l2(pp.download(), O.to_physical(ref_ph, g)) < tol
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_uxw_golden_16(G, fused):
+    """The committed 16^3 fixture: N^, E and Z of the seeded field."""
+    import os
+    say(f"[golden uxw 16] fused={fused}")
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "uxw_16.npz"))
+    grid = G.Grid3D((16, 16, 16))
+    plans = G.FieldTransformPlans(grid)
+    du = G.Field(grid, "spectral", 1).upload(d["u_hat"])
+    dn = G.Field(grid, "spectral", 1)
+    e, z = G.uxw_(dn, du, plans, O.dealias_cutoff(16), fused=fused)
+    assert O.rel_l2(dn.download(), d["n_hat"]) < 1e-12
+    assert abs(e - float(d["energy"])) < 1e-12 * float(d["energy"])
+    assert abs(z - float(d["enstrophy"])) < 1e-12 * float(d["enstrophy"])
+
+
+def test_ops_golden_8x7x6(G):
+    import os
+    say("[golden ops 8x7x6]")
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ops_8x7x6.npz"))
+    grid = G.Grid3D((8, 7, 6))
+    plans = G.FieldTransformPlans(grid)
+    phys = G.Field(grid, "physical", 1).upload(d["phys"])
+    spec = G.to_spectral(phys, plans)
+    assert O.rel_l2(spec.download(), d["spec"]) < 1e-12
+    assert O.rel_l2(G.curl(spec, plans).download(), d["curl"]) < 1e-12
+    pr = G.Field(grid, "spectral", 1)
+    G.helmholtz_projection_(pr, spec, plans)
+    assert O.rel_l2(pr.download(), d["proj"]) < 1e-12
+    assert abs(G.inner(spec, spec, plans) - float(d["inner"])) < 1e-12 * float(d["inner"])
+
+
 def test_refusals(G):
     say("[refusals]")
     grid = G.Grid3D((8, 8, 8))
