@@ -517,6 +517,133 @@ def uxw_(n_hat: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | Non
     return None
 
 
+class Batch:
+    """Many independent (field, operation) items run as one graph launch.
+
+    ``add_scalar(a_hat)`` -> (grad a, dealiased a*a); ``add_vector(u_hat)`` ->
+    (grad u, dealiased u x w).  Outputs are allocated here and returned; all
+    buffers stay put until the batch is closed."""
+
+    SCALAR, VECTOR = 0, 1
+
+    def __init__(self, nstreams: int = 4, autotune: bool = True):
+        h = C.c_void_p()
+        check(load().gbg_batch_create(int(nstreams), 1 if autotune else 0, C.byref(h)))
+        self.handle = h.value
+        self.keep = []
+
+    def _add(self, kind, f: Field, kc):
+        g = f.grid
+        if kc is None:
+            kc = min(dealias_cutoff(n, L) for n, L in zip(g.grid_size, g.domain_size))
+        out0 = Field(g, "spectral", f.order + 1, f.dtype)
+        out1 = Field(g, "spectral", f.order, f.dtype)
+        n = (C.c_int * 3)(*g.grid_size)
+        L = (C.c_double * 3)(*[float(v) for v in g.domain_size])
+        check(load().gbg_batch_add(self.handle, kind, n, L, f.dtype.itemsize, f.data, out0.data,
+                                   out1.data, float(kc)))
+        self.keep += [f, out0, out1]
+        return out0, out1
+
+    def add_scalar(self, a_hat: Field, kc=None):
+        _need(a_hat, "spectral", 0)
+        return self._add(self.SCALAR, a_hat, kc)
+
+    def add_vector(self, u_hat: Field, kc=None):
+        _need(u_hat, "spectral", 1)
+        return self._add(self.VECTOR, u_hat, kc)
+
+    def instantiate(self, stream=None):
+        check(load().gbg_batch_instantiate(self.handle, _sh(stream)))
+
+    def launch(self, stream=None):
+        check(load().gbg_batch_launch(self.handle, _sh(stream)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().gbg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over the C ABI's gbg_server (see
+    include/gbg.h).  ``submit`` returns a ticket at once; ``wait`` blocks for
+    that job and returns its latency in ms."""
+
+    TO_SPECTRAL, TO_PHYSICAL, CURL, INVERSE_LAPLACIAN, DEALIASED_PRODUCT, UXW = range(6)
+
+    def __init__(self, hbm_budget_bytes: int, slots: int = 4):
+        h = C.c_void_p()
+        check(load().gbg_server_create(C.c_size_t(int(hbm_budget_bytes)), int(slots), C.byref(h)))
+        self.handle = h.value
+        self.inflight = {}
+
+    def submit(self, kind: int, out: Field, in0: Field, in1: Field | None = None,
+               kc: float | None = None) -> int:
+        g = in0.grid
+        if kc is None:
+            kc = min(dealias_cutoff(n, L) for n, L in zip(g.grid_size, g.domain_size))
+        j = _lib.GbgJob()
+        j.kind = int(kind)
+        j.n = (C.c_int * 3)(*g.grid_size)
+        j.L = (C.c_double * 3)(*[float(v) for v in g.domain_size])
+        j.elem_bytes = in0.dtype.itemsize
+        j.ncomp = in0.ncomp
+        j.in0 = in0.data
+        j.in1 = in1.data if in1 is not None else None
+        j.out = out.data
+        j.kc = float(kc)
+        t = C.c_ulonglong(0)
+        check(load().gbg_server_submit(self.handle, C.byref(j), C.byref(t)))
+        self.inflight[t.value] = (out, in0, in1)   # keep buffers alive
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        ms = C.c_double(0.0)
+        try:
+            check(load().gbg_server_wait(self.handle, C.c_ulonglong(ticket), C.byref(ms)))
+        finally:
+            self.inflight.pop(ticket, None)
+        return ms.value
+
+    def stats(self):
+        p50, p99, tput = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_ulonglong()
+        check(load().gbg_server_stats(self.handle, C.byref(p50), C.byref(p99), C.byref(tput),
+                                      C.byref(n)))
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "jobs_per_s": tput.value,
+                "completed": n.value}
+
+    def resident(self):
+        b, k = C.c_size_t(), C.c_int()
+        check(load().gbg_server_resident(self.handle, C.byref(b), C.byref(k)))
+        return b.value, k.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().gbg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(plans: FieldTransformPlans, stream=None) -> float:
+    """Tune the plan's per-axis pass choice; returns the forward + inverse ms."""
+    ms = C.c_double(0.0)
+    check(load().gbg_plan_autotune(plans.handle, _sh(stream), C.byref(ms)))
+    return ms.value
+
+
 def dealiased_product_(p_hat: Field, a_hat: Field, b_hat: Field, plans: FieldTransformPlans,
                        kc: float | None = None, stream=None) -> Field:
     """``p^ = M F(F^-1(M a^) F^-1(M b^))`` for scalar fields."""

@@ -18,6 +18,14 @@ STATUS_NAMES = {0: "GBG_OK", 1: "GBG_E_ARG", 2: "GBG_E_DIM", 3: "GBG_E_UNSUPPORT
 
 GBG_UXW_UNFUSED = 1
 
+class GbgJob(C.Structure):
+    """struct gbg_job of include/gbg.h."""
+
+    _fields_ = [("kind", C.c_int), ("n", C.c_int * 3), ("L", C.c_double * 3),
+                ("elem_bytes", C.c_int), ("ncomp", C.c_int), ("in0", C.c_void_p),
+                ("in1", C.c_void_p), ("out", C.c_void_p), ("kc", C.c_double)]
+
+
 vp = C.c_void_p
 st = C.c_int
 sz = C.c_size_t
@@ -59,6 +67,18 @@ SIGNATURES = {
     "gbg_inner_spectral": (st, [vp, vp, vp, i32, dp, vp]),
     "gbg_uxw": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
     "gbg_dealiased_product": (st, [vp, vp, vp, vp, dbl, vp]),
+    "gbg_plan_autotune": (st, [vp, vp, dp]),
+    "gbg_batch_create": (st, [i32, i32, C.POINTER(vp)]),
+    "gbg_batch_add": (st, [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_double), i32, vp, vp, vp, dbl]),
+    "gbg_batch_instantiate": (st, [vp, vp]),
+    "gbg_batch_launch": (st, [vp, vp]),
+    "gbg_batch_destroy": (st, [vp]),
+    "gbg_server_create": (st, [sz, i32, C.POINTER(vp)]),
+    "gbg_server_submit": (st, [vp, vp, C.POINTER(C.c_ulonglong)]),
+    "gbg_server_wait": (st, [vp, C.c_ulonglong, dp]),
+    "gbg_server_stats": (st, [vp, dp, dp, dp, C.POINTER(C.c_ulonglong)]),
+    "gbg_server_resident": (st, [vp, C.POINTER(sz), C.POINTER(C.c_int)]),
+    "gbg_server_destroy": (st, [vp]),
     "gbg_plan_set_profiling": (st, [vp, i32]),
     "gbg_plan_stage_times": (st, [vp, dp, C.POINTER(C.c_int)]),
 }

@@ -186,6 +186,10 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
     const char* env = std::getenv("GBG_GENERIC_FFT");
     p->generic_only = (env && env[0] == '1') ? 1 : 0;
   }
+  for (int a = 0; a < 3; ++a) {
+    p->use_fast[a] = 1;
+    p->bsel[a] = 0;
+  }
   p->nh = n[0] / 2 + 1;
   p->M = M;
   p->P = (size_t)n[0] * n[1] * n[2];

@@ -282,11 +282,12 @@ static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* i
     d.se = ny * nh;
   }
   d.cstride = (long long)p->M;
-  d.B = pick_b(ap.n, Lds<T>::kCap);
+  d.B = p->bsel[axis] ? p->bsel[axis] : pick_b(ap.n, Lds<T>::kCap);
   d.bshift = ilog2(d.B);
   d.ld = ap.n + 1;
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(p->tw[axis]);
-  if (!p->generic_only && try_fast_c2c<T>(ap.n, dir, in, out, d, ncomp, tw, scale, s))
+  if (!p->generic_only && p->use_fast[axis] &&
+      try_fast_c2c<T>(ap.n, dir, in, out, d, ncomp, tw, scale, s))
     return check_launch("c2c pass");
   dim3 grid((unsigned)((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
   if (dir < 0)
@@ -302,8 +303,9 @@ static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, 
   const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
   const cpx<T>* twx = reinterpret_cast<const cpx<T>*>(p->tw[0]);
   const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
-  if (p->generic_only || !try_fast_x<T>(nx, -1, phys, spec, nrows, rg, twx, T(1), s)) {
-    const int B = pick_b(nx, Lds<T>::kCap);
+  if (p->generic_only || !p->use_fast[0] ||
+      !try_fast_x<T>(nx, -1, phys, spec, nrows, rg, twx, T(1), s)) {
+    const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
     dim3 grid((unsigned)((pairs + B - 1) / B));
     hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
@@ -326,8 +328,9 @@ static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, 
   const T scale = T(1.0 / ((double)p->n[0] * (double)p->n[1] * (double)p->n[2]));
   const cpx<T>* twx = reinterpret_cast<const cpx<T>*>(p->tw[0]);
   const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
-  if (p->generic_only || !try_fast_x<T>(nx, 1, work, phys, nrows, rg, twx, scale, s)) {
-    const int B = pick_b(nx, Lds<T>::kCap);
+  if (p->generic_only || !p->use_fast[0] ||
+      !try_fast_x<T>(nx, 1, work, phys, nrows, rg, twx, scale, s)) {
+    const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
     dim3 grid((unsigned)((pairs + B - 1) / B));
     hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,

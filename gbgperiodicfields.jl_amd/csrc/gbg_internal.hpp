@@ -92,6 +92,11 @@ struct gbg_plan {
   int device;
   int generic_only;  // GBG_GENERIC_FFT=1 at creation: mixed-radix LDS passes only
   gbg::AxisPlan ax[3];
+  // per-axis transform choice (set by gbg_plan_autotune): use_fast = take the
+  // register-resident pass where the length allows it; bsel = pencils per
+  // workgroup of the mixed-radix pass (0 = the default choice)
+  int use_fast[3];
+  int bsel[3];
   // FP64 per-axis tables (device): derivative family and squared family
   double* kd[3];
   double* ks[3];
@@ -160,5 +165,8 @@ gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsi
                     hipStream_t s);
 gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
                                   double kc, hipStream_t s);
+// work-area bytes pipe_uxw needs on this plan's grid for cutoff kc (reads
+// only the plan's n, L, nh, M, P, elem and generic_only)
+size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags);
 
 }  // namespace gbg

@@ -564,6 +564,17 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   return GBG_OK;
 }
 
+size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags) {
+  const bool fusable = !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
+                       fast_len(p->n[1]) && fast_len(p->n[2]);
+  const size_t cs = (size_t)p->elem * 2, rs = (size_t)p->elem;
+  if (!fusable) return 2 * (3 * p->M * cs) + 2 * (3 * p->P * rs);
+  const Geo g = make_geo(p, kc);
+  const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cs;
+  const size_t b2 = (size_t)6 * g.s2 * cs;
+  return b1 + b2 + sizeof(double) * 2 * ((size_t)g.nyc * g.nhp + fast::kXcd);
+}
+
 gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
                     hipStream_t s) {
   const bool fusable = !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&

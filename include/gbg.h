@@ -143,6 +143,86 @@ gbg_status gbg_uxw(gbg_plan* plan, const void* u_hat, void* n_hat, double kc, do
 gbg_status gbg_dealiased_product(gbg_plan* plan, void* p_hat, const void* a_hat, const void* b_hat,
                                  double kc, void* stream);
 
+/* ---- autotuning ------------------------------------------------------------
+ * Times, per axis, the candidate pass configurations (register-resident pass
+ * for power-of-two lengths; mixed-radix LDS pass under several radix
+ * orderings and pencils-per-workgroup widths) on a scratch vector field and
+ * keeps the fastest in the plan.  Allocates and frees scratch, synchronises
+ * the stream.  *best_ms (may be NULL) = forward + inverse time of the choice.
+ * No counterpart in the reference (FFTW's planner flags play this role,
+ * src/Fields.jl:1352-1362). */
+gbg_status gbg_plan_autotune(gbg_plan* plan, void* stream, double* best_ms);
+
+/* ---- batched form -----------------------------------------------------------
+ * Many independent items, each a field on its own grid and precision, run as
+ * one hipGraph launch.  Item kinds:
+ *   GBG_ITEM_SCALAR_GRAD_PRODUCT: in = a^ (1 spectral component);
+ *     out0 = grad a^ (3 components), out1 = M F(F^-1(M a^)^2) (1 component)
+ *   GBG_ITEM_VECTOR_GRAD_UXW: in = u^ (3 components);
+ *     out0 = grad u^ (9 components, block i + 3 j = (i a_j) u_i), out1 = the
+ *     dealiased u x w term (3 components, as gbg_uxw)
+ * gbg_batch_instantiate assigns items to nstreams streams, creates (and with
+ * autotune_plans != 0, tunes) one plan per grid, precision and stream, runs
+ * every item once, then captures all of them into one graph.  Buffers must
+ * stay valid and in place until gbg_batch_destroy.  No counterpart in the
+ * reference (it launches per call, src/DifferentialOperators.jl:535-543). */
+typedef struct gbg_batch gbg_batch;
+#define GBG_ITEM_SCALAR_GRAD_PRODUCT 0
+#define GBG_ITEM_VECTOR_GRAD_UXW 1
+gbg_status gbg_batch_create(int nstreams, int autotune_plans, gbg_batch** out);
+gbg_status gbg_batch_add(gbg_batch* batch, int kind, const int n[3], const double L[3],
+                         int elem_bytes, const void* in, void* out0, void* out1, double kc);
+gbg_status gbg_batch_instantiate(gbg_batch* batch, void* stream);
+gbg_status gbg_batch_launch(gbg_batch* batch, void* stream);
+gbg_status gbg_batch_destroy(gbg_batch* batch);
+
+/* ---- serving loop -----------------------------------------------------------
+ * Continuous batching of independent jobs.  gbg_server_submit queues a job
+ * and returns at once; a dispatcher thread admits queued jobs in arrival
+ * order as soon as one of `slots` streams is free and the job's plan (per
+ * grid, precision and slot; created on first use, then resident) fits
+ * `hbm_budget` bytes, evicting idle plans least recently used first, so jobs
+ * that arrive while others run overlap on the device.  Job kinds:
+ *   GBG_JOB_TO_SPECTRAL        in0 physical (ncomp) -> out spectral
+ *   GBG_JOB_TO_PHYSICAL        in0 spectral (ncomp, preserved) -> out physical
+ *   GBG_JOB_CURL               in0 spectral vector -> out
+ *   GBG_JOB_INVERSE_LAPLACIAN  in0 spectral (ncomp) -> out
+ *   GBG_JOB_DEALIASED_PRODUCT  in0 a^, in1 b^ (scalars) -> out p^ (cutoff kc)
+ *   GBG_JOB_UXW                in0 u^ -> out N^ (cutoff kc)
+ * Field buffers are the caller's and must stay valid until the job is
+ * waited for.  Each job runs on a plan of its own slot, so its result does
+ * not depend on the other jobs in flight.  No counterpart in the reference
+ * (a consumer-side request loop over the calls above). */
+typedef struct gbg_server gbg_server;
+#define GBG_JOB_TO_SPECTRAL 0
+#define GBG_JOB_TO_PHYSICAL 1
+#define GBG_JOB_CURL 2
+#define GBG_JOB_INVERSE_LAPLACIAN 3
+#define GBG_JOB_DEALIASED_PRODUCT 4
+#define GBG_JOB_UXW 5
+typedef struct gbg_job {
+  int kind;
+  int n[3];
+  double L[3];
+  int elem_bytes;
+  int ncomp;
+  const void* in0;
+  const void* in1;
+  void* out;
+  double kc;
+} gbg_job;
+gbg_status gbg_server_create(size_t hbm_budget, int slots, gbg_server** out);
+gbg_status gbg_server_submit(gbg_server* server, const gbg_job* job, unsigned long long* ticket);
+/* Blocks until the job is done; *latency_ms = completion - submission. */
+gbg_status gbg_server_wait(gbg_server* server, unsigned long long ticket, double* latency_ms);
+/* Latency percentiles over completed jobs and their throughput since the
+ * first submission. */
+gbg_status gbg_server_stats(gbg_server* server, double* p50_ms, double* p99_ms, double* jobs_per_s,
+                            unsigned long long* completed);
+/* Device bytes held by the resident plans and their count. */
+gbg_status gbg_server_resident(gbg_server* server, size_t* bytes, int* plans);
+gbg_status gbg_server_destroy(gbg_server* server);
+
 /* ---- profiling aid --------------------------------------------------------
  * When enabled, fused pipelines record HIP events between their passes on
  * the caller's stream; gbg_plan_stage_times synchronises and returns the
