@@ -85,7 +85,7 @@ def stage_bytes(n, elem):
 
 KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}>", "K2_y_inverse": "k_uxw_y<{t}, {n}, 1>",
                    "K3_x_product": "k_uxw_x<{t}, {n}>", "K4_y_forward": "k_uxw_y<{t}, {n}, -1>",
-                   "K5_z_forward_project": "k_uxw_z_forward<{t}, {n}>"}
+                   "K5_z_forward_project": ("k_uxw_zero_tiles<{t}>", "k_uxw_z_forward<{t}, {n}>")}
 
 
 def measured_traffic(stage, n, dtype):
@@ -97,9 +97,14 @@ def measured_traffic(stage, n, dtype):
     with open(path) as fh:
         data = json.load(fh)
     t = "double" if dtype == np.float64 else "float"
-    k = KERNEL_OF_STAGE[stage].format(t=t, n=n[0])
-    rec = data.get("kernels", {}).get(k)
-    return None if rec is None else int(rec["hbm_bytes"])
+    ks = KERNEL_OF_STAGE[stage]
+    total = 0
+    for k in (ks,) if isinstance(ks, str) else ks:
+        rec = data.get("kernels", {}).get(k.format(t=t, n=n[0]))
+        if rec is None:
+            return None
+        total += int(rec["hbm_bytes"])
+    return total
 
 
 def run_gpu(G, n, dtype, steps, warmup, stream, ev, seed=2001):

@@ -148,6 +148,8 @@ static void destroy_plan(gbg_plan* p) {
   if (p->work) (void)hipFree(p->work);
   if (p->red_part) (void)hipFree(p->red_part);
   if (p->red_out) (void)hipFree(p->red_out);
+  for (int i = 0; i < p->ntset; ++i)
+    if (p->tset[i].dev) (void)hipFree(p->tset[i].dev);
   for (int i = 0; i < kMaxEvents; ++i)
     if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
   delete p;

@@ -80,6 +80,18 @@ struct Tables {
 constexpr int kMaxEvents = 16;
 constexpr int kRedBlocks = 1024;   // fixed reduction grid: deterministic sums
 
+// Output tiles (B consecutive (ky, kx) pencils) of the fused pipeline's last
+// pass for one cutoff: dev[0, nd) lists the tiles that hold a kept (kx, ky)
+// pencil, dev[nd, nd + nt) flags every tile 1 if it is in that list, else 0.
+// Built once per (plan, cutoff) outside any stream capture and kept, so a
+// captured graph may reference it for the plan's lifetime.
+constexpr int kTileSets = 8;
+struct TileSet {
+  double kc;
+  int b, nd, nt;
+  int* dev;
+};
+
 }  // namespace gbg
 
 struct gbg_plan {
@@ -108,6 +120,9 @@ struct gbg_plan {
   // reduction partials and results (FP64)
   double* red_part;
   double* red_out;
+  // tile tables of the fused u x omega pipeline's last pass, one per cutoff
+  gbg::TileSet tset[gbg::kTileSets];
+  int ntset;
   size_t device_bytes;
   // profiling events
   int profiling;

@@ -16,7 +16,8 @@
 //   K3  x-pass:      6 C2R rows -> u x w in registers -> 3 R2C rows (kx <= kx_max)
 //   K4  y-forward:   3 fields, stores the disk ky range only
 //   K5  z-forward:   3 fields -> Leray projection, mask, mean-free -> N^ (full
-//                    layout, zeros outside the mask)
+//                    layout, zeros outside the mask), on the output tiles that
+//                    hold disk pencils; a write-only kernel zeros the other tiles
 // Buffers: W1 = [6][Nz][nyc][nhk] (K1 -> K2), reused as [3][Nz][nyc][nhk]
 // (K4 -> K5); W2 = [6][Nz][Ny][nhk] (K2 -> K3), reused as [3][Nz][Ny][nhk]
 // (K3 -> K4).  nhk = kx_max + 1, nyc = number of kept ky indices.
@@ -84,7 +85,7 @@ __device__ __forceinline__ int ky_of(int kyc, const Geo& g) {
   return kyc < g.ya ? kyc : g.ny - g.nyc + kyc;
 }
 // compact index of a full ky index, or -1 when the mask drops it
-__device__ __forceinline__ int kyc_of(int ky, const Geo& g) {
+__host__ __device__ __forceinline__ int kyc_of(int ky, const Geo& g) {
   if (ky < g.ya) return ky;
   const int c = ky - (g.ny - g.nyc);
   return (c >= g.ya && c < g.nyc) ? c : -1;
@@ -366,18 +367,22 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
 
 // ---- K5: z-forward, Leray projection, mask, mean-free ------------------------
 // Tiles: B consecutive pencils of the output's flattened (ky, kx) index, so
-// the (largest) output stream is written in aligned segments.
+// the (largest) output stream is written in aligned segments.  With a tile
+// list, workgroup i takes tile list[i] (the tiles that hold kept pencils) and
+// k_uxw_zero_tiles writes the others; without one, workgroup i takes tile i
+// and tiles without a kept pencil are written as zeros here.
 template <typename T, int N>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ w1, cpx<T>* __restrict__ nh, Geo g, Tables tb,
-    const cpx<T>* __restrict__ tw, long long ntiles) {
+    const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[2 * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const long long tile = fast::xcd_tile();
+  const long long lt = fast::xcd_tile();
+  const long long tile = (list && lt < ntiles) ? (long long)list[lt] : lt;
   const long long p5l = tile * B + pl;
-  const bool valid = tile < ntiles && p5l < (long long)g.ny * g.nh;
+  const bool valid = lt < ntiles && p5l < (long long)g.ny * g.nh;
   const int p5 = valid ? (int)p5l : 0;
   int kx = 0, ky = 0, kyc = -1;
   double qxy = 0.0;
@@ -433,6 +438,22 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     nh[g.M + o] = keep ? p2 : zero;
     nh[2 * g.M + o] = keep ? p3 : zero;
   }
+}
+
+// The output tiles that hold no kept (kx, ky) pencil: zeros in all kz planes
+// and components, one pencil index per thread, zper kz planes per workgroup row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
+                                                        const int* __restrict__ flag, int b,
+                                                        long long plane, long long M, int nz,
+                                                        int zper) {
+  const long long p5 = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p5 >= plane || flag[p5 / b]) return;
+  const int z0 = blockIdx.y * zper;
+  const int z1 = z0 + zper < nz ? z0 + zper : nz;
+  const cpx<T> zero = mk<T>(T(0), T(0));
+  for (int c = 0; c < 3; ++c)
+    for (int z = z0; z < z1; ++z) nh[c * M + (long long)z * plane + p5] = zero;
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -492,9 +513,61 @@ static Geo make_geo(gbg_plan* p, double kc) {
   return g;
 }
 
+// The plan's tile set for cutoff g.hi2 and tile width b, built on first use.
+// Returns nullptr (the last pass then classifies tiles itself) when the set
+// is not cached and cannot be built now: the stream is being captured, or
+// the plan already holds kTileSets sets.
+static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipStream_t s) {
+  for (int i = 0; i < p->ntset; ++i)
+    if (p->tset[i].kc == kc && p->tset[i].b == b) return &p->tset[i];
+  if (p->ntset >= kTileSets) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  std::vector<double> s1, s2;
+  axis_host(g.nx, p->L[0], true, s1);
+  axis_host(g.ny, p->L[1], false, s2);
+  const long long plane = (long long)g.ny * g.nh;
+  const int nt = (int)((plane + b - 1) / b);
+  // a tile is listed when one of its pencils may be kept; the slack only
+  // lists extra tiles, which the last pass then writes as zeros itself
+  const double hi = g.hi2 * (1.0 + 1e-9);
+  std::vector<int> list, flag(nt, 0);
+  for (int t = 0; t < nt; ++t) {
+    for (long long q = (long long)t * b; q < (long long)(t + 1) * b && q < plane; ++q) {
+      const int ky = (int)(q / g.nh), kx = (int)(q - (long long)ky * g.nh);
+      if (kx < g.nhk && kyc_of(ky, g) >= 0 && s1[kx] + s2[ky] <= hi) {
+        flag[t] = 1;
+        break;
+      }
+    }
+    if (flag[t]) list.push_back(t);
+  }
+  const int nd = (int)list.size();
+  list.insert(list.end(), flag.begin(), flag.end());
+  const size_t bytes = sizeof(int) * list.size();
+  int* dev = nullptr;
+  if (hipMalloc((void**)&dev, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipMemcpy(dev, list.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(dev);
+    return nullptr;
+  }
+  TileSet& ts = p->tset[p->ntset++];
+  ts.kc = kc;
+  ts.b = b;
+  ts.nd = nd;
+  ts.nt = nt;
+  ts.dev = dev;
+  p->device_bytes += bytes;
+  return &ts;
+}
+
 template <typename T>
 static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
-                            hipStream_t s) {
+                            double kc, hipStream_t s) {
   const size_t cb = sizeof(cpx<T>);
   // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
   // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields
@@ -555,9 +628,23 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
-    const long long nt = ((long long)g.ny * g.nh + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2, out,
-                       g, tb, twz, nt);
+    const long long plane = (long long)g.ny * g.nh;
+    const TileSet* ts = tile_set(p, g, C::B, kc, s);
+    if (!ts) {
+      const long long nt = (plane + C::B - 1) / C::B;
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2,
+                         out, g, tb, twz, (const int*)nullptr, nt);
+      return;
+    }
+    if (ts->nd < ts->nt) {
+      constexpr int zper = 8;
+      const dim3 zg((unsigned)((plane + 255) / 256), (unsigned)((g.nz + zper - 1) / zper));
+      hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, C::B,
+                         plane, g.M, g.nz, zper);
+    }
+    if (ts->nd > 0)
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
+                         W2, out, g, tb, twz, (const int*)ts->dev, (long long)ts->nd);
   });
   if ((st = check_launch("uxw K5"))) return st;
   prof_mark(p, s);
@@ -581,8 +668,9 @@ gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsi
                        fast_len(p->n[1]) && fast_len(p->n[2]);
   if (!fusable) return pipe_uxw_composed(p, u_hat, n_hat, kc, s);
   const Geo g = make_geo(p, kc);
-  if (p->elem == 8) return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, s);
-  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, s);
+  if (p->elem == 8)
+    return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, kc, s);
+  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, s);
 }
 
 gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
