@@ -79,10 +79,26 @@ constexpr int num_stages(int n) {
   return s;
 }
 
+// LDS slot of pencil element e: PD = 0 stores e at e; PD = 1 leaves one
+// empty slot after every 8 (slot e + e/8, a pencil spans N + N/8 slots).  With
+// the pencil strides of Cfg this spreads the stride-8 writes of the first
+// stage and the unit-stride accesses of all stages over distinct banks (32
+// banks of 4 bytes; 16-byte accesses are served 8 lanes per cycle, 8-byte
+// accesses 16).  Offsets that are multiples of 8 map linearly, so every
+// access is a per-thread base plus a compile-time offset.
+template <int PD>
+__device__ __forceinline__ int slot(int e) {
+  return PD ? e + (e >> 3) : e;
+}
+template <int PD>
+constexpr int slot_step(int c) {
+  return PD ? c + c / 8 : c;
+}
+
 // Stage (R, LS) of the ping-pong schedule: exchange k (k = 0, 1, ...) writes
 // buffer k mod 2 and the next stage reads it, so one barrier per exchange
 // suffices -- a buffer is rewritten only two exchanges after it was read.
-template <typename T, int N, int E, int LS, int DIR, int SYNC, int W>
+template <typename T, int N, int E, int LS, int DIR, int SYNC, int W, int PD>
 __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                        const cpx<T>* __restrict__ tw) {
   constexpr int R = StageR<N, LS>::R;
@@ -92,6 +108,8 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
   constexpr int TP = N / E;
   constexpr bool FIRST = (LS == 1);
   constexpr bool LAST = (LS * R == N);
+  static_assert(!PD || ((FIRST || NR % 8 == 0) && (LAST || LS % 8 == 0 || (LS == 1 && R == 8))),
+                "slot offsets must be multiples of 8");
   cpx<T>* const rd = W == 0 ? p1 : p0;   // written by the previous stage
   cpx<T>* const pen = W == 0 ? p0 : p1;  // written by this stage
   cpx<T> a[NB][R];
@@ -104,8 +122,9 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int j = t + TP * b;
+      const cpx<T>* q = rd + slot<PD>(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[b][r] = rd[j + r * NR];
+      for (int r = 0; r < R; ++r) a[b][r] = q[r * slot_step<PD>(NR)];
     }
   }
 #pragma unroll
@@ -126,31 +145,28 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
     for (int b = 0; b < NB; ++b) {
       const int j = t + TP * b;
       const int k = j % LS;
-      cpx<T>* o = pen + (j - k) * R + k;
+      // LS = 1: slot(8 j + r) = 9 j + r; LS >= 8: slot(o + r LS) = slot(o) + r slot_step(LS)
+      cpx<T>* o = pen + slot<PD>((j - k) * R + k);
 #pragma unroll
-      for (int r = 0; r < R; ++r) o[r * LS] = a[b][r];
+      for (int r = 0; r < R; ++r) o[r * (LS == 1 ? 1 : slot_step<PD>(LS))] = a[b][r];
     }
     lds_sync<SYNC>();
-    stages<T, N, E, LS * R, DIR, SYNC, 1 - W>(v, p0, p1, t, tw);
+    stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD>(v, p0, p1, t, tw);
   }
 }
 
 // In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  p0, p1 are
-// this pencil's two LDS regions (N elements each).  On return p0 may be
-// written at once (its last read was followed by a barrier); p1 may not.
-template <typename T, int N, int E, int DIR, int SYNC>
+// this pencil's two LDS regions (slot<PD> layout, at least N elements).  On
+// return p0 may be written at once (its last read was followed by a barrier);
+// p1 may not.
+template <typename T, int N, int E, int DIR, int SYNC, int PD>
 __device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                     const cpx<T>* __restrict__ tw) {
-  stages<T, N, E, 1, DIR, SYNC, 0>(v, p0, p1, t, tw);
+  stages<T, N, E, 1, DIR, SYNC, 0, PD>(v, p0, p1, t, tw);
   // with an odd number of exchanges the last one read p0
   if constexpr ((num_stages(N) - 1) % 2 == 1) lds_sync<SYNC>();
 }
 
-// LDS pencil stride: padding spreads the pencils of a tile over the banks.
-template <typename T, int N>
-struct Ld {
-  static constexpr int value = N + (sizeof(T) == 8 ? 2 : 4);
-};
 
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
 // Giving each XCD a contiguous run of logical tiles makes neighbouring tiles,
@@ -173,11 +189,17 @@ struct Cfg {
   static constexpr int B0 = WG / TP;
   static constexpr int B = B0 < 1 ? 1 : (B0 > 64 ? 64 : B0);
   static constexpr int NT = B * TP;
-  static constexpr int LD = Ld<T, N>::value;
+  // strided passes: consecutive lanes hold different pencils of a tile; an
+  // odd pencil stride puts them on distinct banks
+  static constexpr int PD = N >= 1024 ? 1 : 0;
+  static constexpr int LD = PD ? N + N / 8 + 2 : N + 1;
   // x pass: row-pair units per workgroup
   static constexpr int U = TP >= 256 ? 1 : 256 / TP;
   static constexpr int XNT = U * TP;
   static constexpr int XSYNC = TP <= 64 ? 1 : 0;
+  // x pass: consecutive lanes hold consecutive elements of one pencil
+  static constexpr int XPD = 1;
+  static constexpr int XLD = N + N / 8;
 };
 
 
@@ -187,32 +209,58 @@ struct Cfg {
 //
 // c2r_pair_load: Z_k = A_k + i B_k over the Hermitian completion of the two
 // half-spectrum rows ra, rb (entries kx >= nhk are zero), with the
-// self-conjugate entries taken real as a c2r transform treats them.
+// self-conjugate entries taken real as a c2r transform treats them.  Split in
+// c2r_pair_fetch (the raw entries a[m], b[m] this thread needs, issued as
+// loads) and c2r_pair_combine (the completion), so a caller can issue the
+// loads of one row pair before transforming another.
 template <typename T, int N, int E, class G>
-__device__ __forceinline__ void c2r_pair_load(cpx<T> (&v)[E], const cpx<T>* __restrict__ ra,
-                                              const cpx<T>* __restrict__ rb, bool oka, bool okb,
-                                              int t, const G& g) {
+__device__ __forceinline__ void c2r_pair_fetch(cpx<T> (&a)[E], cpx<T> (&b)[E],
+                                               const cpx<T>* __restrict__ ra,
+                                               const cpx<T>* __restrict__ rb, bool oka, bool okb,
+                                               int t, const G& g) {
+  using C = Cfg<T, N>;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + C::TP * m;
+    const int kk = k >= g.nh ? N - k : k;
+    a[m] = mk<T>(T(0), T(0));
+    b[m] = mk<T>(T(0), T(0));
+    if (kk < g.nhk) {
+      if (oka) a[m] = ra[kk];
+      if (okb) b[m] = rb[kk];
+    }
+  }
+}
+
+template <typename T, int N, int E, class G>
+__device__ __forceinline__ void c2r_pair_combine(cpx<T> (&v)[E], const cpx<T> (&a)[E],
+                                                 const cpx<T> (&b)[E], int t, const G& g) {
   using C = Cfg<T, N>;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + C::TP * m;
     const bool up = k >= g.nh;
     const int kk = up ? N - k : k;
-    cpx<T> a = mk<T>(T(0), T(0)), b = mk<T>(T(0), T(0));
-    if (kk < g.nhk) {
-      if (oka) a = ra[kk];
-      if (okb) b = rb[kk];
-    }
+    cpx<T> x = a[m], y = b[m];
     if (kk == 0 || kk == g.nyq_x) {
-      a.y = T(0);
-      b.y = T(0);
+      x.y = T(0);
+      y.y = T(0);
     }
     if (up) {
-      a.y = -a.y;
-      b.y = -b.y;
+      x.y = -x.y;
+      y.y = -y.y;
     }
-    v[m] = mk<T>(a.x - b.y, a.y + b.x);
+    v[m] = mk<T>(x.x - y.y, x.y + y.x);
   }
+}
+
+template <typename T, int N, int E, class G>
+__device__ __forceinline__ void c2r_pair_load(cpx<T> (&v)[E], const cpx<T>* __restrict__ ra,
+                                              const cpx<T>* __restrict__ rb, bool oka, bool okb,
+                                              int t, const G& g) {
+  cpx<T> a[E], b[E];
+  c2r_pair_fetch<T, N, E>(a, b, ra, rb, oka, okb, t, g);
+  c2r_pair_combine<T, N, E>(v, a, b, t, g);
 }
 
 // forward-transformed Z = a + i b -> A_k, B_k for k < nhk, stored to ra / rb

@@ -137,7 +137,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_fast_c2c(const cpx<T>* in, 
 #pragma unroll
   for (int m = 0; m < E; ++m)
     v[m] = valid ? in[base + (long long)(t + TP * m) * d.se] : mk<T>(T(0), T(0));
-  fast::fft<T, N, E, DIR, 0>(v, lds + pl * LD, lds + (B + pl) * LD, t, tw);
+  fast::fft<T, N, E, DIR, 0, C::PD>(v, lds + pl * LD, lds + (B + pl) * LD, t, tw);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < E; ++m) out[base + (long long)(t + TP * m) * d.se] = cscale(v[m], scale);
@@ -155,7 +155,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __rest
                                                                long long nrows, RowG g,
                                                                const cpx<T>* __restrict__ tw) {
   using C = Cfg<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::LD;
+  constexpr int E = C::E, TP = C::TP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __rest
   }
   cpx<T>* pen = lds + un * LD;
   cpx<T>* pen1 = lds + (C::U + un) * LD;
-  fast::fft<T, N, E, -1, C::XSYNC>(v, pen, pen1, t, tw);
+  fast::fft<T, N, E, -1, C::XSYNC, C::XPD>(v, pen, pen1, t, tw);
   fast::r2c_pair_store<T, N, E>(v, pen, out + (ok0 ? r0 * g.nh : 0), out + (ok1 ? r1 * g.nh : 0),
                                 ok0, ok1, t, g);
 }
@@ -180,7 +180,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_c2r_x(const cpx<T>* _
                                                                RowG g, const cpx<T>* __restrict__ tw,
                                                                T scale) {
   using C = Cfg<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::LD;
+  constexpr int E = C::E, TP = C::TP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_c2r_x(const cpx<T>* _
   cpx<T> v[E];
   fast::c2r_pair_load<T, N, E>(v, in + (ok0 ? r0 * g.nh : 0), in + (ok1 ? r1 * g.nh : 0), ok0, ok1,
                                t, g);
-  fast::fft<T, N, E, 1, C::XSYNC>(v, lds + un * LD, lds + (C::U + un) * LD, t, tw);
+  fast::fft<T, N, E, 1, C::XSYNC, C::XPD>(v, lds + un * LD, lds + (C::U + un) * LD, t, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int x = t + TP * m;

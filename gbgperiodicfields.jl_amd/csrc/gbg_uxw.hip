@@ -176,7 +176,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       }
     }
     if (f + 1 < 6) load(f + 1);
-    fast::fft<T, N, E, 1, 0>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, 1, 0, C::PD>(v, pen, pen1, t, tw);
     if (disk) {
       cpx<T>* o = w1 + f * g.s1 + p1;
 #pragma unroll
@@ -288,7 +288,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = nxt[m];
     if (f + 1 < NF) load(f + 1, nxt);
-    fast::fft<T, N, E, DIR, 0>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, DIR, 0, C::PD>(v, pen, pen1, t, tw);
     if (DIR > 0) {
       if (valid) {
         cpx<T>* o = dst + f * g.s2 + fbase;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
                                                              const cpx<T>* __restrict__ tw, T scale,
                                                              long long ntiles) {
   using C = Cfg<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::LD, S = C::XSYNC;
+  constexpr int E = C::E, TP = C::TP, LD = C::XLD, S = C::XSYNC;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long tile = fast::xcd_tile();
@@ -334,7 +334,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
       cpx<T> z[E];
       fast::c2r_pair_load<T, N, E>(z, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok,
                                    ok, t, g);
-      fast::fft<T, N, E, 1, S>(z, pen, pen1, t, tw);
+      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         ph[2 * pi][m] = z[m].x * scale;
@@ -354,13 +354,13 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
         n3b[m] = n3v;
       v[m] = mk<T>(n1, n2);
     }
-    fast::fft<T, N, E, -1, S>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, tw);
     fast::r2c_pair_store<T, N, E>(v, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
   }
   cpx<T> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = mk<T>(n3a[m], n3b[m]);
-  fast::fft<T, N, E, -1, S>(v, pen, pen1, t, tw);
+  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, tw);
   fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhp : 0),
                                 w3 + 2 * g.s2 + (ok1 ? r1 * g.nhp : 0), ok0, ok1, t, g);
 }
@@ -416,7 +416,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     for (int m = 0; m < E; ++m) n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0>(n[c], pen, pen1, t, tw);
+  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0, C::PD>(n[c], pen, pen1, t, tw);
   if (!valid) return;
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
 #pragma unroll
