@@ -81,6 +81,15 @@ struct Geo {
 
 using fast::Cfg;
 
+// With OPAQUE, the same pointer made opaque to the optimiser: table loads
+// behind it stay where they are written instead of being hoisted (and held in
+// registers) across the whole kernel.
+template <bool OPAQUE, typename P>
+__device__ __forceinline__ P* fresh(P* p) {
+  if (OPAQUE) asm volatile("" : "+s"(p));
+  return p;
+}
+
 __device__ __forceinline__ int ky_of(int kyc, const Geo& g) {
   return kyc < g.ya ? kyc : g.ny - g.nyc + kyc;
 }
@@ -176,7 +185,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       }
     }
     if (f + 1 < 6) load(f + 1);
-    fast::fft<T, N, E, 1, 0, C::PD>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, 1, 0, C::PD>(v, pen, pen1, t, fresh<sizeof(T) == 4>(tw));
     if (disk) {
       cpx<T>* o = w1 + f * g.s1 + p1;
 #pragma unroll
@@ -305,8 +314,18 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 }
 
 // ---- K3: x pass -- C2R of 6 rows, u x w, R2C of 3 rows ------------------------
-// One unit = a pair of rows (r0, r1) of the flattened (y, z) index; the two
-// rows' n3 share one complex transform.  Rows have pitch nhp.
+// One unit = a pair of rows (r0, r1) of the flattened (y, z) index; rows have
+// pitch nhp.  Per row the six fields go through three complex transforms in
+// the pairs (u1, w1), (u2, w2), (u3, w3), and the cross product is built as
+// they arrive, so at most five real rows are held in registers:
+//   after (u2, w2):  n3 = u1 w2 - u2 w1
+//   after (u3, w3):  n1 = u2 w3 - u3 w2,  n2 = u3 w1 - u1 w3
+// (n1, n2) of a row share one forward transform, n3 of the two rows another;
+// n3 of the first row waits in LDS.  When a pencil sits in one wave the
+// exchanges need no second LDS buffer: a wave's LDS operations execute in
+// order.  In FP64, table loads go through fresh() so the optimiser does not
+// hoist the twiddles of every stage out of the loops and hold them in
+// registers (which would spill); FP32 has the registers to spare.
 template <typename T, int N>
 __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
                                                              cpx<T>* __restrict__ w3, Geo g,
@@ -314,53 +333,67 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
                                                              long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, LD = C::XLD, S = C::XSYNC;
-  __shared__ cpx<T> lds[2 * C::U * LD];
+  constexpr int NBUF = S ? 1 : 2;
+  __shared__ cpx<T> lds[NBUF * C::U * LD];
+  // n3 of the first row: in LDS when the pencil sits in one wave (a single
+  // exchange buffer leaves room), else in registers
+  __shared__ T held[S ? C::U * E * TP : 1];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long tile = fast::xcd_tile();
   const long long nrows = (long long)g.ny * g.nz;
   const long long r0 = 2 * (tile * C::U + un), r1 = r0 + 1;
   const bool ok0 = tile < ntiles && r0 < nrows, ok1 = tile < ntiles && r1 < nrows;
   cpx<T>* pen = lds + un * LD;
-  cpx<T>* pen1 = lds + (C::U + un) * LD;
-  T n3a[E], n3b[E];
+  cpx<T>* pen1 = S ? pen : lds + (C::U + un) * LD;
+  T* hold = held + (S ? un * E * TP + t : 0);
+  T N3[E], n3a[E];
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
-    const long long r = h ? r1 : r0;
+    const long long ro = (h ? ok1 : ok0) ? (h ? r1 : r0) * g.nhp : 0;
     const bool ok = h ? ok1 : ok0;
-    const long long ro = ok ? r * g.nhp : 0;
-    T ph[6][E];
+    T U1[E], W1[E], U2[E], W2[E];
 #pragma unroll
-    for (int pi = 0; pi < 3; ++pi) {
+    for (int q = 0; q < 3; ++q) {
       cpx<T> z[E];
-      fast::c2r_pair_load<T, N, E>(z, w2 + (2 * pi) * g.s2 + ro, w2 + (2 * pi + 1) * g.s2 + ro, ok,
-                                   ok, t, g);
-      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, tw);
+      fast::c2r_pair_load<T, N, E>(z, w2 + q * g.s2 + ro, w2 + (q + 3) * g.s2 + ro, ok, ok, t, g);
+      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
+      if (q == 0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        ph[2 * pi][m] = z[m].x * scale;
-        ph[2 * pi + 1][m] = z[m].y * scale;
+        for (int m = 0; m < E; ++m) {
+          U1[m] = z[m].x * scale;
+          W1[m] = z[m].y * scale;
+        }
+      } else if (q == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          U2[m] = z[m].x * scale;
+          W2[m] = z[m].y * scale;
+          N3[m] = U1[m] * W2[m] - U2[m] * W1[m];
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const T u3 = z[m].x * scale, w3v = z[m].y * scale;
+          z[m] = mk<T>(U2[m] * w3v - u3 * W2[m], u3 * W1[m] - U1[m] * w3v);
+        }
+        fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
+        fast::r2c_pair_store<T, N, E>(z, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
       }
     }
-    // (u x w)_i = u_j w_k - u_k w_j; ph = u1, u2, u3, w1, w2, w3
-    cpx<T> v[E];
+    if (h == 0) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const T n1 = ph[1][m] * ph[5][m] - ph[2][m] * ph[4][m];
-      const T n2 = ph[2][m] * ph[3][m] - ph[0][m] * ph[5][m];
-      const T n3v = ph[0][m] * ph[4][m] - ph[1][m] * ph[3][m];
-      if (h == 0)
-        n3a[m] = n3v;
-      else
-        n3b[m] = n3v;
-      v[m] = mk<T>(n1, n2);
+      for (int m = 0; m < E; ++m) {
+        if (S)
+          hold[m * TP] = N3[m];
+        else
+          n3a[m] = N3[m];
+      }
     }
-    fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, tw);
-    fast::r2c_pair_store<T, N, E>(v, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
   }
   cpx<T> v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = mk<T>(n3a[m], n3b[m]);
-  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, tw);
+  for (int m = 0; m < E; ++m) v[m] = mk<T>(S ? hold[m * TP] : n3a[m], N3[m]);
+  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
   fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhp : 0),
                                 w3 + 2 * g.s2 + (ok1 ? r1 * g.nhp : 0), ok0, ok1, t, g);
 }
@@ -432,11 +465,12 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T> p1 = csub(n[0][m], cscale(lon, a1));
     const cpx<T> p2 = csub(n[1][m], cscale(lon, a2));
     const cpx<T> p3 = csub(n[2][m], cscale(lon, a3));
-    const cpx<T> zero = mk<T>(T(0), T(0));
     const long long o = (long long)kz * plane + p5;
-    nh[o] = keep ? p1 : zero;
-    nh[g.M + o] = keep ? p2 : zero;
-    nh[2 * g.M + o] = keep ? p3 : zero;
+    // component-wise selects (a select of whole complex values is lowered
+    // through a private array in FP64)
+    nh[o] = mk<T>(keep ? p1.x : T(0), keep ? p1.y : T(0));
+    nh[g.M + o] = mk<T>(keep ? p2.x : T(0), keep ? p2.y : T(0));
+    nh[2 * g.M + o] = mk<T>(keep ? p3.x : T(0), keep ? p3.y : T(0));
   }
 }
 
