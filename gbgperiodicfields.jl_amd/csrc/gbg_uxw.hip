@@ -321,42 +321,117 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 //   after (u2, w2):  n3 = u1 w2 - u2 w1
 //   after (u3, w3):  n1 = u2 w3 - u3 w2,  n2 = u3 w1 - u1 w3
 // (n1, n2) of a row share one forward transform, n3 of the two rows another;
-// n3 of the first row waits in LDS.  When a pencil sits in one wave the
-// exchanges need no second LDS buffer: a wave's LDS operations execute in
-// order.  In FP64, table loads go through fresh() so the optimiser does not
-// hoist the twiddles of every stage out of the loops and hold them in
-// registers (which would spill); FP32 has the registers to spare.
+// n3 of the first row waits in LDS.  When a pencil sits in one wave (S) the
+// exchanges need no second LDS buffer (a wave's LDS operations execute in
+// order), the twiddle table is read from LDS, and with STG the half-spectrum
+// entries of the next field pair are copied from HBM straight into a per-wave
+// LDS staging area (global_load_lds) while the current pair is transformed.
+// Without S, FP64 table loads go through fresh() so the optimiser does not
+// hoist the twiddles of every stage and hold them in registers.
 template <typename T, int N>
+struct XStage {
+  static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
+  static constexpr int CH = sizeof(T) == 8 ? 3 : 2;            // copies per field row
+  static constexpr int TPE = Cfg<T, N>::TP * EPL;              // entries per copy per unit
+  static constexpr int CAP = CH * TPE;                         // staged entries per row
+  static constexpr int WAVE_ELEMS = 2 * CH * 64 * EPL;         // staging elements per wave
+  static constexpr int ALL8 = (fast::num_stages(N) > 0) && ([] {
+    int n = N;
+    while (n % 8 == 0) n /= 8;
+    return n == 1;
+  })();
+  static constexpr int TWN = ALL8 ? N / 2 : N;                 // twiddle entries the stages read
+};
+
+template <typename T, int N, bool STG>
 __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
                                                              cpx<T>* __restrict__ w3, Geo g,
                                                              const cpx<T>* __restrict__ tw, T scale,
                                                              long long ntiles) {
   using C = Cfg<T, N>;
+  using X = XStage<T, N>;
   constexpr int E = C::E, TP = C::TP, LD = C::XLD, S = C::XSYNC;
+  static_assert(!STG || S, "staging needs a pencil per wave");
   constexpr int NBUF = S ? 1 : 2;
+  constexpr int NW = C::XNT / 64;
   __shared__ cpx<T> lds[NBUF * C::U * LD];
-  // n3 of the first row: in LDS when the pencil sits in one wave (a single
-  // exchange buffer leaves room), else in registers
   __shared__ T held[S ? C::U * E * TP : 1];
+  __shared__ cpx<T> twl[S ? X::TWN : 1];
+  __shared__ cpx<T> stg[STG ? NW * X::WAVE_ELEMS : 1];
+  if (S) {
+    for (int i = threadIdx.x; i < X::TWN; i += C::XNT) twl[i] = tw[i];
+    __syncthreads();
+  }
+  auto twp = [&]() { return S ? (const cpx<T>*)twl : fresh<sizeof(T) == 8>(tw); };
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long tile = fast::xcd_tile();
   const long long nrows = (long long)g.ny * g.nz;
   const long long r0 = 2 * (tile * C::U + un), r1 = r0 + 1;
   const bool ok0 = tile < ntiles && r0 < nrows, ok1 = tile < ntiles && r1 < nrows;
+  const long long ro0 = ok0 ? r0 * g.nhp : 0, ro1 = ok1 ? r1 * g.nhp : 0;
   cpx<T>* pen = lds + un * LD;
   cpx<T>* pen1 = S ? pen : lds + (C::U + un) * LD;
   T* hold = held + (S ? un * E * TP + t : 0);
+  // staging: this wave's area holds field a then field b, CH copies each; copy
+  // c of unit u' (lanes u' TP .. u' TP + TP - 1) lands at c 64 EPL + u' TPE
+  cpx<T>* sw = stg + (STG ? (threadIdx.x / 64) * X::WAVE_ELEMS : 0);
+  const int uw = (threadIdx.x % 64) / TP;
+  auto issue = [&](long long ro, int q) {
+    if constexpr (STG) {
+#pragma unroll
+      for (int c = 0; c < X::CH; ++c) {
+        const int e0 = c * X::TPE + t * X::EPL;
+        const int e = e0 < g.nhk ? e0 : 0;   // entries past nhk are never read back
+        __builtin_amdgcn_global_load_lds(
+            (const void __attribute__((address_space(1)))*)(w2 + q * g.s2 + ro + e),
+            (void __attribute__((address_space(3)))*)(sw + c * 64 * X::EPL), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const void __attribute__((address_space(1)))*)(w2 + (q + 3) * g.s2 + ro + e),
+            (void __attribute__((address_space(3)))*)(sw + (X::CH + c) * 64 * X::EPL), 16, 0, 0);
+      }
+    }
+  };
+  // the raw entries of pair q of a row: from the staging area or from HBM
+  auto fetch = [&](cpx<T> (&a)[E], cpx<T> (&b)[E], long long ro, bool ok, int q) {
+    if constexpr (STG) {
+      __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's copies have landed
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + TP * m;
+        const int kk = k >= g.nh ? N - k : k;
+        a[m] = mk<T>(T(0), T(0));
+        b[m] = mk<T>(T(0), T(0));
+        if (kk < g.nhk) {
+          const int idx = (kk / X::TPE) * 64 * X::EPL + uw * X::TPE + kk % X::TPE;
+          a[m] = sw[idx];
+          b[m] = sw[X::CH * 64 * X::EPL + idx];
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging reads done before reuse
+    } else {
+      fast::c2r_pair_fetch<T, N, E>(a, b, w2 + q * g.s2 + ro, w2 + (q + 3) * g.s2 + ro, ok, ok, t, g);
+    }
+  };
   T N3[E], n3a[E];
+  issue(ro0, 0);
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
-    const long long ro = (h ? ok1 : ok0) ? (h ? r1 : r0) * g.nhp : 0;
+    const long long ro = h ? ro1 : ro0;
     const bool ok = h ? ok1 : ok0;
     T U1[E], W1[E], U2[E], W2[E];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       cpx<T> z[E];
-      fast::c2r_pair_load<T, N, E>(z, w2 + q * g.s2 + ro, w2 + (q + 3) * g.s2 + ro, ok, ok, t, g);
-      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
+      {
+        cpx<T> a[E], b[E];
+        fetch(a, b, ro, ok, q);
+        if (q < 2)
+          issue(ro, q + 1);
+        else if (h == 0)
+          issue(ro1, 0);
+        fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
+      }
+      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
       if (q == 0) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -376,7 +451,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
           const T u3 = z[m].x * scale, w3v = z[m].y * scale;
           z[m] = mk<T>(U2[m] * w3v - u3 * W2[m], u3 * W1[m] - U1[m] * w3v);
         }
-        fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
+        fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, twp());
         fast::r2c_pair_store<T, N, E>(z, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
       }
     }
@@ -393,9 +468,8 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
   cpx<T> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = mk<T>(S ? hold[m * TP] : n3a[m], N3[m]);
-  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, fresh<sizeof(T) == 8>(tw));
-  fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + (ok0 ? r0 * g.nhp : 0),
-                                w3 + 2 * g.s2 + (ok1 ? r1 * g.nhp : 0), ok0, ok1, t, g);
+  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, twp());
+  fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + ro0, w3 + 2 * g.s2 + ro1, ok0, ok1, t, g);
 }
 
 // ---- K5: z-forward, Leray projection, mask, mean-free ------------------------
@@ -645,8 +719,14 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     using C = Cfg<T, N>;
     const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
     const long long nt = (pairs + C::U - 1) / C::U;
-    hipLaunchKernelGGL((k_uxw_x<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2, W1, g,
-                       twx, T(1.0 / P), nt);
+    // stage the next pair through LDS when a pencil sits in one wave and the
+    // kept kx range fits the staging area
+    if (C::XSYNC && g.nhk <= XStage<T, N>::CAP)
+      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
+                         s, W2, W1, g, twx, T(1.0 / P), nt);
+    else
+      hipLaunchKernelGGL((k_uxw_x<T, N, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2,
+                         W1, g, twx, T(1.0 / P), nt);
   });
   if ((st = check_launch("uxw K3"))) return st;
   prof_mark(p, s);

@@ -191,7 +191,7 @@ def test_operators(G, n, L, dtype):
 
 UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), ((48, 32, 40), None),
              ((64, 64, 128), None), ((128, 128, 32), None), ((64, 64, 64), 5.0), ((64, 64, 64), 1000.0),
-             ((32, 32, 64), 7.3)]
+             ((32, 32, 64), 7.3), ((512, 32, 32), 170.5), ((256, 64, 32), 60.0)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
