@@ -193,6 +193,82 @@ def run_batch(G, stream, ev, steps, warmup, nitems=96, seed=4001):
     return res
 
 
+def run_hbm(G, stream, ev, steps, nbytes=1 << 30):
+    """Streaming rates the device sustains, for reading the pass rooflines:
+    the runtime's device-to-device copy kernel (nbytes read + nbytes written)
+    and its fill kernel (nbytes written), timed with events on the stream."""
+    lib = G.load()
+    src, dst = C.c_void_p(), C.c_void_p()
+    G.check(lib.gbg_malloc(C.byref(src), nbytes))
+    G.check(lib.gbg_malloc(C.byref(dst), nbytes))
+    res = {"bytes": nbytes}
+    try:
+        for name, call, moved in (
+                ("fill", lambda: lib.gbg_memset(dst.value, 1, nbytes, stream.handle), nbytes),
+                ("copy", lambda: lib.gbg_memcpy_d2d(dst.value, src.value, nbytes, stream.handle),
+                 2 * nbytes)):
+            for _ in range(2):
+                G.check(call())
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, stream.handle)
+            for _ in range(steps):
+                G.check(call())
+            ev.record(e1, stream.handle)
+            ms = ev.elapsed(e0, e1) / steps
+            res[name] = {"ms": round(ms, 4), "GBs": round(moved / (ms * 1e-3) / 1e9, 1)}
+            print(f"[hbm] {name}: {res[name]}", file=sys.stderr, flush=True)
+    finally:
+        lib.gbg_free(src.value)
+        lib.gbg_free(dst.value)
+    return res
+
+
+def run_ops(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
+    """The operators either side of the nonlinear term at the headline size:
+    the exponential kernel on a vector field (single- and two-operand forms),
+    the radial spectrum and the reality projection, each with its algorithmic
+    bytes (component = M complex elements) and fraction of the HBM peak."""
+    from gbg_amd import workload
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = workload.synthetic_field(n, 2001, vector=True)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dv = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dw = G.Field(grid, "spectral", 1, dtype)
+    del u
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    comp = M * 2 * np.dtype(dtype).itemsize
+    cases = [
+        ("exponential_convolve (vector, in place)", 6 * comp,
+         lambda: G.exponential_convolve_(dv, dv, plans, coefficient=-1e-6, stream=stream)),
+        ("exponential_convolve two-operand (vector)", 9 * comp,
+         lambda: G.exponential_convolve_(dw, 1.0, du, 0.5, dv, plans, coefficient=-1e-6,
+                                         stream=stream)),
+        ("radial_spectrum (vector)", 3 * comp, lambda: G.radial_spectrum(du, plans, stream=stream)),
+        ("reality_projection (vector, in place)", None,
+         lambda: G.reality_projection_(dv, plans, stream=stream)),
+    ]
+    res = {}
+    for name, nbytes, call in cases:
+        for _ in range(warmup):
+            call()
+        stream.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream.handle)
+        for _ in range(steps):
+            call()
+        ev.record(e1, stream.handle)
+        ms = ev.elapsed(e0, e1) / steps
+        rec = {"ms": round(ms, 4)}
+        if nbytes:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            rec.update({"bytes_alg": int(nbytes), "achieved_GBs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        res[name] = rec
+        print(f"[ops] {name}: {rec}", file=sys.stderr, flush=True)
+    return res
+
+
 def smooth_sizes(lo=32, hi=512):
     out = []
     for n in range(lo, hi + 1, 2):
@@ -332,9 +408,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--form", default="uxw", choices=["uxw", "batch", "serve"],
+    ap.add_argument("--form", default="uxw", choices=["uxw", "batch", "serve", "hbm", "ops"],
                     help="uxw: the headline 512^3 term (default); batch: the 96-field batched "
-                         "graph form; serve: the Poisson request stream")
+                         "graph form; serve: the Poisson request stream; hbm: the streaming "
+                         "copy/fill rates of the device; ops: exponential kernel, radial "
+                         "spectrum and reality projection at --size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device twice (a plain copy kernel of known "
                          "size, the scale for the HBM byte counters)")
@@ -366,6 +444,31 @@ def main():
                "config": {"workload": "96-field batch, grids drawn from " +
                           ", ".join("x".join(map(str, s)) for s in BATCH_SHAPES)},
                "batch": res}
+        print(json.dumps(out), flush=True)
+        return
+    if args.form == "ops":
+        res = run_ops(G, stream, ev, args.steps, args.warmup, n, dtype)
+        first = res["exponential_convolve (vector, in place)"]
+        out = {"metric": f"{n[0]}^3 exponential_convolve (vector) GB/s + radial spectrum, "
+                         "reality projection", "value": first["achieved_GBs"], "unit": "GB/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": first["ms"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": args.dtype,
+               "data": "synthetic (seeded k^-5/3 solenoidal spectrum, seed 2001)",
+               "config": {"workload": f"{n[0]}x{n[1]}x{n[2]} vector field operators"},
+               "roofline": {"bound": "hbm", "achieved": first["achieved_GBs"],
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"],
+                            "traffic": None},
+               "ops": res}
+        print(json.dumps(out), flush=True)
+        return
+    if args.form == "hbm":
+        res = run_hbm(G, stream, ev, args.steps)
+        out = {"metric": "HBM streaming rate: runtime copy kernel (1 GiB read + 1 GiB write)",
+               "value": res["copy"]["GBs"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": 2, "ms_per_step": res["copy"]["ms"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "device buffers",
+               "config": {"workload": "1 GiB device-to-device copy and 1 GiB fill"}, "hbm": res}
         print(json.dumps(out), flush=True)
         return
     if args.form == "serve":

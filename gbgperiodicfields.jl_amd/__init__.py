@@ -493,6 +493,113 @@ def l2norm(f: Field, plans: FieldTransformPlans, stream=None) -> float:
     return math.sqrt(inner(f, f, plans, stream))
 
 
+def radial_spectrum(f: Field, *args, stream=None):
+    """``radial_spectrum(f, m)`` / ``radial_spectrum(f, g, m)`` (``src/Reductions.jl:1309-1455``):
+    returns ``(shells, spectrum)``, FP64 numpy arrays; the shell abscissa is
+    ``m dk`` with ``dk = 2 pi / max L``."""
+    if len(args) == 1:
+        g, plans = None, args[0]
+    elif len(args) == 2:
+        g, plans = args
+    else:
+        raise ArgumentError(1, "radial_spectrum(f, plans) or radial_spectrum(f, g, plans)")
+    _need(f, "spectral")
+    _with_plan(f, plans)
+    if g is not None:
+        _need(g, "spectral")
+        _same_domain(f, g)
+        if f.order != g.order:
+            raise ArgumentError(1, f"radial_spectrum(f, g, m) needs equal tensor orders "
+                                   f"(got {f.order} and {g.order})")
+    n, dk = C.c_int(0), C.c_double(0.0)
+    check(load().gbg_radial_shells(plans.handle, C.byref(n), C.byref(dk)))
+    spec = np.zeros(n.value, dtype=np.float64)
+    check(load().gbg_radial_spectrum(plans.handle, f.data, None if g is None else g.data, f.ncomp,
+                                     spec.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                     _sh(stream)))
+    return np.arange(n.value) * dk.value, spec
+
+
+# ---- exponential kernel (src/Convolution.jl:1197-1795) -----------------------------
+
+def _offset_args(offset, f: Field):
+    if offset is None:
+        return None
+    _need(offset, "spectral", 0)
+    _same_domain(offset, f)
+    return offset.data
+
+
+def exponential_convolve_(dst: Field, *args, coefficient: float, offset: Field | None = None,
+                          offset_scale: float = 1.0, stream=None) -> Field:
+    """``exponential_convolve!(dst, f, m[, alpha, beta]; coefficient, offset, offset_scale)``
+    and the two-operand ``exponential_convolve!(dst, alpha, a, beta, b, m; ...)``
+    (``src/Convolution.jl:1750``): the kernel ``exp(c |k|^2 + s w)`` evaluated per
+    mode, never stored."""
+    _need(dst, "spectral")
+    if len(args) >= 1 and isinstance(args[0], Field):
+        f, plans = args[0], args[1]
+        alpha, beta = (args[2], args[3]) if len(args) == 4 else (1.0, 0.0)
+        if len(args) not in (2, 4):
+            raise ArgumentError(1, "exponential_convolve!(dst, f, m[, alpha, beta])")
+        _need(f, "spectral", dst.order)
+        _with_plan(f, plans)
+        _same_domain(dst, f)
+        check(load().gbg_exponential_convolve(plans.handle, dst.data, f.data, f.ncomp,
+                                              float(coefficient), _offset_args(offset, f),
+                                              float(offset_scale), float(alpha), float(beta),
+                                              _sh(stream)))
+        return dst
+    if len(args) != 5:
+        raise ArgumentError(1, "exponential_convolve!(dst, alpha, a, beta, b, m)")
+    alpha, a, beta, b, plans = args
+    for x in (a, b):
+        _need(x, "spectral", dst.order)
+        _same_domain(dst, x)
+    _with_plan(a, plans)
+    check(load().gbg_exponential_combine(plans.handle, dst.data, float(alpha), a.data, float(beta),
+                                         b.data, a.ncomp, float(coefficient),
+                                         _offset_args(offset, a), float(offset_scale), _sh(stream)))
+    return dst
+
+
+def exponential_convolve(f: Field, plans: FieldTransformPlans, *, coefficient: float,
+                         offset: Field | None = None, offset_scale: float = 1.0,
+                         stream=None) -> Field:
+    """Allocating ``exponential_convolve``."""
+    return exponential_convolve_(f.similar(), f, plans, coefficient=coefficient, offset=offset,
+                                 offset_scale=offset_scale, stream=stream)
+
+
+# ---- reality condition (src/Fields.jl:2858-3094) -----------------------------------
+
+def reality_residual(f: Field, plans: FieldTransformPlans, stream=None) -> float:
+    """``reality_residual(f)``: the largest ``|f(k) - conj f(-k)|`` on the
+    self-conjugate planes (``src/Fields.jl:2918-2928``)."""
+    _need(f, "spectral")
+    _with_plan(f, plans)
+    out = C.c_double(0.0)
+    check(load().gbg_reality_residual(plans.handle, f.data, f.ncomp, C.byref(out), _sh(stream)))
+    return out.value
+
+
+def reality_projection_(dst: Field, *args, stream=None) -> Field:
+    """``reality_projection!(f, plans)`` in place or ``reality_projection!(dst, src,
+    plans)`` (``src/Fields.jl:2940-3094``)."""
+    if len(args) == 1:
+        src, plans = dst, args[0]
+    elif len(args) == 2:
+        src, plans = args
+    else:
+        raise ArgumentError(1, "reality_projection!(f, plans) or (dst, src, plans)")
+    _need(src, "spectral")
+    _need(dst, "spectral", src.order)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_reality_projection(plans.handle, dst.data, src.data, src.ncomp, _sh(stream)))
+    return dst
+
+
 # ---- fused pipelines ---------------------------------------------------------------
 
 def uxw_(n_hat: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | None = None,

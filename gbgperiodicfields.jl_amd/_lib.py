@@ -65,6 +65,12 @@ SIGNATURES = {
     "gbg_cross": (st, [vp, vp, vp, vp, vp]),
     "gbg_multiply": (st, [vp, vp, vp, vp, vp]),
     "gbg_inner_spectral": (st, [vp, vp, vp, i32, dp, vp]),
+    "gbg_radial_shells": (st, [vp, C.POINTER(C.c_int), dp]),
+    "gbg_radial_spectrum": (st, [vp, vp, vp, i32, dp, i32, vp]),
+    "gbg_exponential_convolve": (st, [vp, vp, vp, i32, dbl, vp, dbl, dbl, dbl, vp]),
+    "gbg_exponential_combine": (st, [vp, vp, dbl, vp, dbl, vp, i32, dbl, vp, dbl, vp]),
+    "gbg_reality_residual": (st, [vp, vp, i32, dp, vp]),
+    "gbg_reality_projection": (st, [vp, vp, vp, i32, vp]),
     "gbg_uxw": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
     "gbg_dealiased_product": (st, [vp, vp, vp, vp, dbl, vp]),
     "gbg_plan_autotune": (st, [vp, vp, dp]),

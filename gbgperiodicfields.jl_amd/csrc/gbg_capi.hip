@@ -563,6 +563,113 @@ gbg_status gbg_inner_spectral(gbg_plan* p, const void* f, const void* g, int nco
   GBG_GUARD_END
 }
 
+gbg_status gbg_radial_shells(gbg_plan* p, int* nshells, double* dk) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!nshells || !dk) return fail(GBG_E_ARG, "null argument");
+  radial_shells(p, nshells, dk);
+  return GBG_OK;
+  GBG_GUARD_END
+}
+
+gbg_status gbg_radial_spectrum(gbg_plan* p, const void* f, const void* g, int ncomp,
+                               double* spectrum, int nshells, void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!f || !spectrum) return fail(GBG_E_ARG, "null argument");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  int n = 0;
+  double dk = 0.0;
+  radial_shells(p, &n, &dk);
+  if (nshells != n) return fail(GBG_E_DIM, "nshells does not match the plan's shell count");
+  hipStream_t s = as_stream(stream);
+  if ((st = ensure_work(p, radial_work_bytes(p)))) return st;
+  double* part = (double*)p->work;
+  double* out = part + (radial_work_bytes(p) / sizeof(double) - (size_t)n);
+  if ((st = op_radial_spectrum(p, f, g, ncomp, n, out, part, s))) return st;
+  if ((st = check_hip(hipMemcpyAsync(spectrum, out, sizeof(double) * n, hipMemcpyDeviceToHost, s),
+                      "spectrum d2h")))
+    return st;
+  return check_hip(hipStreamSynchronize(s), "spectrum sync");
+  GBG_GUARD_END
+}
+
+gbg_status gbg_exponential_convolve(gbg_plan* p, void* dst, const void* f, int ncomp,
+                                    double coefficient, const void* offset, double offset_scale,
+                                    double alpha, double beta, void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!dst || !f) return fail(GBG_E_ARG, "null field buffer");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  if (!std::isfinite(coefficient) || !std::isfinite(offset_scale) || !std::isfinite(alpha) ||
+      !std::isfinite(beta))
+    return fail(GBG_E_ARG, "coefficient, offset_scale, alpha and beta must be finite");
+  const size_t b = ncomp * p->M * 2 * p->elem, b1 = p->M * 2 * p->elem;
+  if ((st = same_or_disjoint(dst, f, b))) return st;
+  // the offset may be a component of dst (read per mode before any write there)
+  if (offset && offset != dst && overlaps(dst, b, offset, b1))
+    return fail(GBG_E_ALIAS, "destination partially overlaps the offset");
+  return op_exponential(p, dst, f, ncomp, coefficient, offset, offset_scale, alpha, beta,
+                        as_stream(stream));
+  GBG_GUARD_END
+}
+
+gbg_status gbg_exponential_combine(gbg_plan* p, void* dst, double alpha, const void* a,
+                                   double beta, const void* b, int ncomp, double coefficient,
+                                   const void* offset, double offset_scale, void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!dst || !a || !b) return fail(GBG_E_ARG, "null field buffer");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  if (!std::isfinite(coefficient) || !std::isfinite(offset_scale) || !std::isfinite(alpha) ||
+      !std::isfinite(beta))
+    return fail(GBG_E_ARG, "coefficient, offset_scale, alpha and beta must be finite");
+  const size_t n = ncomp * p->M * 2 * p->elem, b1 = p->M * 2 * p->elem;
+  if ((st = same_or_disjoint(dst, a, n)) || (st = same_or_disjoint(dst, b, n))) return st;
+  if (offset && offset != dst && overlaps(dst, n, offset, b1))
+    return fail(GBG_E_ALIAS, "destination partially overlaps the offset");
+  return op_exponential_combine(p, dst, alpha, a, beta, b, ncomp, coefficient, offset,
+                                offset_scale, as_stream(stream));
+  GBG_GUARD_END
+}
+
+gbg_status gbg_reality_residual(gbg_plan* p, const void* spec, int ncomp, double* residual,
+                                void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!spec || !residual) return fail(GBG_E_ARG, "null argument");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  hipStream_t s = as_stream(stream);
+  if ((st = op_reality_residual(p, spec, ncomp, p->red_out + 3, s))) return st;
+  if ((st = check_hip(hipMemcpyAsync(residual, p->red_out + 3, sizeof(double),
+                                     hipMemcpyDeviceToHost, s), "residual d2h")))
+    return st;
+  return check_hip(hipStreamSynchronize(s), "residual sync");
+  GBG_GUARD_END
+}
+
+gbg_status gbg_reality_projection(gbg_plan* p, void* dst, const void* src, int ncomp,
+                                  void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!dst || !src) return fail(GBG_E_ARG, "null field buffer");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  const size_t b = ncomp * p->M * 2 * p->elem;
+  if ((st = same_or_disjoint(dst, src, b))) return st;
+  hipStream_t s = as_stream(stream);
+  if (dst != src &&
+      (st = check_hip(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s), "projection copy")))
+    return st;
+  return op_reality_project(p, dst, ncomp, s);
+  GBG_GUARD_END
+}
+
 gbg_status gbg_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, double* energy,
                    double* enstrophy, unsigned flags, void* stream) {
   GBG_GUARD_BEGIN

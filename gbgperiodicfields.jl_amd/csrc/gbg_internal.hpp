@@ -175,6 +175,23 @@ gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hip
 // Parseval inner into p->red_out[slot] (device); no synchronisation
 gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int slot, hipStream_t s);
 
+// exponential kernel, radial spectrum, reality condition (gbg_next.hip)
+gbg_status op_exponential(gbg_plan* p, void* dst, const void* src, int ncomp, double coefficient,
+                          const void* offset, double offset_scale, double alpha, double beta,
+                          hipStream_t s);
+gbg_status op_exponential_combine(gbg_plan* p, void* dst, double alpha, const void* a, double beta,
+                                  const void* b, int ncomp, double coefficient, const void* offset,
+                                  double offset_scale, hipStream_t s);
+void radial_shells(const gbg_plan* p, int* nshells, double* dk);
+size_t radial_work_bytes(const gbg_plan* p);
+// part: radial_work_bytes of device scratch; out_dev: nsh doubles on the device
+gbg_status op_radial_spectrum(gbg_plan* p, const void* f, const void* g, int ncomp, int nsh,
+                              double* out_dev, double* part, hipStream_t s);
+gbg_status op_reality_project(gbg_plan* p, void* spec, int ncomp, hipStream_t s);
+// the residual into *out_dev (device); uses the plan's reduction partials
+gbg_status op_reality_residual(gbg_plan* p, const void* spec, int ncomp, double* out_dev,
+                               hipStream_t s);
+
 // pipelines (gbg_uxw.hip)
 gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
                     hipStream_t s);

@@ -127,6 +127,49 @@ gbg_status gbg_multiply(gbg_plan* plan, void* dst, const void* a, const void* b,
 gbg_status gbg_inner_spectral(gbg_plan* plan, const void* f, const void* g, int ncomp, double* out,
                               void* stream);
 
+/* Radial (shell-binned) spectrum, radial_spectrum(f[, g], m)
+ * (src/Reductions.jl:1309-1455): shell m collects the modes with
+ * round(|k|/dk) = m (ties to even), dk = 2 pi / max L, Parseval-weighted
+ * sum_c |f_c|^2 (or Re f_c conj g_c when g is not NULL) over (Nx Ny Nz)^2.
+ * gbg_radial_shells gives the shell count and width (:1334-1338);
+ * gbg_radial_spectrum writes nshells FP64 values to the host array
+ * `spectrum` (summed in a fixed order: deterministic) and synchronises. */
+gbg_status gbg_radial_shells(gbg_plan* plan, int* nshells, double* dk);
+gbg_status gbg_radial_spectrum(gbg_plan* plan, const void* f, const void* g, int ncomp,
+                               double* spectrum, int nshells, void* stream);
+
+/* ---- exponential kernel (src/Convolution.jl:1197-1795) --------------------- */
+/* exponential_convolve!(dst, f, m, alpha, beta; coefficient, offset,
+ * offset_scale) (:1750; _exponential_convolution_kernel! :1326-1346):
+ *   dst = alpha exp(c |k|^2 + s w) f  (+ beta dst when beta != 0),
+ * |k|^2 the true squared wavenumber, w = offset (NULL, or one spectral
+ * component of the plan's precision; a complex w gives a complex factor).
+ * The factor is evaluated per mode in FP64 and never stored.  dst may be f or
+ * offset. */
+gbg_status gbg_exponential_convolve(gbg_plan* plan, void* dst, const void* f, int ncomp,
+                                    double coefficient, const void* offset, double offset_scale,
+                                    double alpha, double beta, void* stream);
+/* The two-operand form exponential_convolve!(dst, alpha, a, beta, b, m; ...)
+ * (_exponential_combination_kernel! :1349-1362): dst = exp(c |k|^2 + s w)
+ * (alpha a + beta b) in one pass.  dst may be a, b or offset. */
+gbg_status gbg_exponential_combine(gbg_plan* plan, void* dst, double alpha, const void* a,
+                                   double beta, const void* b, int ncomp, double coefficient,
+                                   const void* offset, double offset_scale, void* stream);
+
+/* ---- reality condition (src/Fields.jl:2858-3094) ---------------------------- */
+/* reality_residual (:2918-2928): the largest |f(k) - conj f(-k)| over the
+ * kx = 0 plane and, for even Nx, the Nyquist plane, every component.  Host
+ * FP64 result; synchronises. */
+gbg_status gbg_reality_residual(gbg_plan* plan, const void* spec, int ncomp, double* residual,
+                                void* stream);
+/* reality_projection!(dst, src) / reality_projection!(f) (:2940-3094): each
+ * stored conjugate pair on those planes becomes its mean (x + conj y)/2 and
+ * the conjugate of that; a self-paired corner its real part; every other
+ * coefficient untouched.  dst == src projects in place (one or two planes);
+ * otherwise src is copied first. */
+gbg_status gbg_reality_projection(gbg_plan* plan, void* dst, const void* src, int ncomp,
+                                  void* stream);
+
 /* ---- fused pipelines ------------------------------------------------------ */
 /* The dealiased rotational nonlinear term of examples/HIST_CPU.jl:229-247:
  *   N^ = Q M P F( F^-1(M u^) x F^-1(curl M u^) )

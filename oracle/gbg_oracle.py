@@ -404,6 +404,117 @@ def dealiased_product(a_hat: np.ndarray, b_hat: np.ndarray, grid: Grid3D, kc: fl
 
 
 # ---------------------------------------------------------------------------
+# Exponential kernel  (src/Convolution.jl:1197-1795)
+# ---------------------------------------------------------------------------
+
+def exponential_factor(m: Multipliers, coefficient: float, offset: np.ndarray | None = None,
+                       offset_scale: float = 1.0) -> np.ndarray:
+    """Per-mode transfer function ``exp(c |k|^2 [+ s w])`` with ``|k|^2 =
+    (k1^2 + k2^2) + k3^2`` from the squared family (``_exponential_exponent``,
+    ``src/Convolution.jl:1289-1294``; kernel ``:1326-1346``).  A complex offset
+    gives a complex exponential, as the reference's broadcast does."""
+    e = coefficient * m.ksq()
+    if offset is None:
+        return np.exp(e)
+    return np.exp(e + offset_scale * offset)
+
+
+def exponential_convolve(f: np.ndarray, m: Multipliers, coefficient: float, alpha: float = 1.0,
+                         beta: float = 0.0, dst: np.ndarray | None = None,
+                         offset: np.ndarray | None = None, offset_scale: float = 1.0) -> np.ndarray:
+    """``exponential_convolve!`` single-operand form (``src/Convolution.jl:1750``;
+    ``_exponential_convolution_kernel!`` ``:1326-1346``, scalar broadcast
+    ``:1211-1232``): ``(alpha exp(...)) f``, plus ``beta dst`` only when beta != 0."""
+    fac = alpha * exponential_factor(m, coefficient, offset, offset_scale)
+    out = fac * f
+    if beta != 0.0:
+        out = out + beta * dst
+    return out
+
+
+def exponential_combine(alpha: float, a: np.ndarray, beta: float, b: np.ndarray, m: Multipliers,
+                        coefficient: float, offset: np.ndarray | None = None,
+                        offset_scale: float = 1.0) -> np.ndarray:
+    """``exponential_convolve!(dst, alpha, a, beta, b, m; ...)`` (the two-operand
+    form, ``_exponential_combination_kernel!`` ``src/Convolution.jl:1349-1362``,
+    scalar broadcast ``:1237-1255``): ``exp(...) (alpha a + beta b)``."""
+    return exponential_factor(m, coefficient, offset, offset_scale) * (alpha * a + beta * b)
+
+
+# ---------------------------------------------------------------------------
+# Radial (shell-binned) spectrum  (src/Reductions.jl:1309-1455)
+# ---------------------------------------------------------------------------
+
+def radial_shells(m: Multipliers) -> tuple[int, float]:
+    """Shell count and width of ``_radial_spectrum_bin`` (``src/Reductions.jl:1334-1338``):
+    ``dk = 2 pi / max L``; ``round(sqrt(max k1^2 + max k2^2 + max k3^2) / dk) + 1``
+    shells (``round`` ties to even)."""
+    dk = TWO_PI / max(m.grid.L)
+    corner = float(m.s1.max()) + float(m.s2.max()) + float(m.s3.max())
+    return int(np.rint(math.sqrt(corner) / dk)) + 1, dk
+
+
+def radial_spectrum(f: np.ndarray, m: Multipliers, g: np.ndarray | None = None):
+    """``radial_spectrum(f[, g], m)`` (``_spectral_shell_bin_kernel!``
+    ``src/Reductions.jl:1309-1327``, ``_radial_spectrum_bin`` ``:1330-1358``):
+    shell ``round(|k| / dk)`` (ties to even), Parseval weight 1 on the kx = 0 and
+    even-Nx Nyquist planes and 2 elsewhere, density ``sum_c |f_c|^2`` (or
+    ``Re(f_c conj(g_c))``), each shell divided by ``(Nx Ny Nz)^2``.  Returns
+    ``(shells, spectrum)``."""
+    grid = m.grid
+    nsh, dk = radial_shells(m)
+    k2 = (m.s1[None, None, :] + m.s2[None, :, None]) + m.s3[:, None, None]
+    shell = np.rint(np.sqrt(k2) / dk).astype(np.int64)
+    ff = f.reshape((-1,) + grid.spectral_shape)
+    if g is None:
+        dens = np.sum(np.abs(ff) ** 2, axis=0)
+    else:
+        gg = g.reshape((-1,) + grid.spectral_shape)
+        dens = np.sum((ff * np.conj(gg)).real, axis=0)
+    w = parseval_weight(grid)
+    spec = np.bincount(shell.ravel(), weights=(w * dens).ravel(), minlength=nsh)[:nsh]
+    spec = spec / float(grid.npoints) ** 2
+    return np.arange(nsh) * dk, spec
+
+
+# ---------------------------------------------------------------------------
+# Reality condition on the self-conjugate planes  (src/Fields.jl:2858-3094)
+# ---------------------------------------------------------------------------
+
+def _reality_partner(plane: np.ndarray) -> np.ndarray:
+    """The entry at (-kz, -ky) for every (kz, ky) of a (..., Nz, Ny) plane."""
+    nz, ny = plane.shape[-2], plane.shape[-1]
+    jz = (-np.arange(nz)) % nz
+    jy = (-np.arange(ny)) % ny
+    return plane[..., jz, :][..., :, jy]
+
+
+def reality_residual(spec: np.ndarray, grid: Grid3D) -> float:
+    """``reality_residual`` (``src/Fields.jl:2858-2869, 2918-2928``): the largest
+    ``|f(k) - conj(f(-k))|`` over the kx = 0 plane and, for even Nx, the Nyquist
+    plane, every component."""
+    planes = [0] + ([grid.nh - 1] if grid.n[0] % 2 == 0 else [])
+    r = 0.0
+    for p in planes:
+        pl = spec[..., :, :, p]
+        r = max(r, float(np.max(np.abs(pl - np.conj(_reality_partner(pl))))))
+    return r
+
+
+def reality_projection(spec: np.ndarray, grid: Grid3D) -> np.ndarray:
+    """``reality_projection!`` (``src/Fields.jl:2940-3094``; ``_project_pair!``
+    ``:2949-2958``): every entry of a self-conjugate plane becomes
+    ``(f(k) + conj(f(-k))) / 2`` (its partner the conjugate of that, a
+    self-paired corner its real part); all other entries are untouched."""
+    out = spec.copy()
+    planes = [0] + ([grid.nh - 1] if grid.n[0] % 2 == 0 else [])
+    for p in planes:
+        pl = spec[..., :, :, p]
+        out[..., :, :, p] = (pl + np.conj(_reality_partner(pl))) / 2.0
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Synthetic inputs (seeded; the workload generator)
 # ---------------------------------------------------------------------------
 
