@@ -91,9 +91,9 @@ __global__ __launch_bounds__(kNT) void k_exponential_combine(cpx<T>* dst, const 
 // instead of the reference's atomics: along an x row |k| grows with kx, so a wave reads 64
 // consecutive kx of a row, adds up equal-shell runs with a segmented scan, and the run ends
 // add into the wave's own LDS histogram (distinct shells, rows in a fixed order).  The
-// workgroup writes the sum of its four histograms, a second kernel adds the workgroups'
-// rows in order.
-constexpr int kRadialBlocks = 1024;
+// workgroup writes the sum of its four histograms to part[shell][workgroup], a second
+// kernel adds each shell's row in a fixed order.
+constexpr int kRadialBlocks = 4096;
 constexpr int kRadialMaxShells = 2048;   // 4 waves x 2048 x 8 B = 64 KiB of LDS
 
 template <typename T>
@@ -145,16 +145,24 @@ __global__ __launch_bounds__(kNT) void k_radial_bin(const cpx<T>* __restrict__ f
   }
   __syncthreads();
   for (int s = threadIdx.x; s < nsh; s += kNT)
-    part[(size_t)blockIdx.x * nsh + s] = ((hist[s] + hist[nsh + s]) + hist[2 * nsh + s]) + hist[3 * nsh + s];
+    part[(size_t)s * gridDim.x + blockIdx.x] =
+        ((hist[s] + hist[nsh + s]) + hist[2 * nsh + s]) + hist[3 * nsh + s];
 }
 
+// one workgroup per shell: strided partial sums, then a fixed tree
 __global__ __launch_bounds__(kNT) void k_radial_final(const double* __restrict__ part, int nparts,
-                                                      int nsh, double norm, double* __restrict__ out) {
-  const int s = blockIdx.x * kNT + threadIdx.x;
-  if (s >= nsh) return;
+                                                      double norm, double* __restrict__ out) {
+  __shared__ double red[kNT];
+  const double* row = part + (size_t)blockIdx.x * nparts;
   double acc = 0.0;
-  for (int b = 0; b < nparts; ++b) acc += part[(size_t)b * nsh + s];
-  out[s] = acc / norm;
+  for (int b = threadIdx.x; b < nparts; b += kNT) acc += row[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = kNT / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0] / norm;
 }
 
 // ---- reality condition ---------------------------------------------------------------
@@ -309,7 +317,7 @@ gbg_status op_radial_spectrum(gbg_plan* p, const void* f, const void* g, int nco
   gbg_status st = check_launch("radial spectrum bins");
   if (st) return st;
   const double N = (double)p->n[0] * (double)p->n[1] * (double)p->n[2];
-  hipLaunchKernelGGL(k_radial_final, grid_of(nsh), kNT, 0, s, part, (int)nb, nsh, N * N, out_dev);
+  hipLaunchKernelGGL(k_radial_final, nsh, kNT, 0, s, part, (int)nb, N * N, out_dev);
   return check_launch("radial spectrum final");
 }
 
