@@ -130,6 +130,51 @@ function inner(f::AbstractFieldSpectral, g::AbstractFieldSpectral, plan::GBGPlan
     return out[]
 end
 
+# ---- exponential kernel (src/Convolution.jl:1750) ----------------------------------------
+optr(offset) = offset === nothing ? C_NULL : dptr(offset)
+exponential_convolve!(dst, f, plan::GBGPlan, α::Real = 1, β::Real = 0;
+                      coefficient::Real, offset = nothing, offset_scale::Real = 1) =
+    (check(ccall((:gbg_exponential_convolve, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Ptr{Cvoid}, Cdouble, Cdouble,
+                  Cdouble, Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(f), ncomp(f), coefficient, optr(offset),
+                 offset_scale, α, β, C_NULL)); dst)
+exponential_convolve!(dst, α::Real, a, β::Real, b, plan::GBGPlan;
+                      coefficient::Real, offset = nothing, offset_scale::Real = 1) =
+    (check(ccall((:gbg_exponential_combine, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Cint, Cdouble,
+                  Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+                 plan.handle, dptr(dst), α, dptr(a), β, dptr(b), ncomp(a), coefficient,
+                 optr(offset), offset_scale, C_NULL)); dst)
+
+# ---- radial spectrum (src/Reductions.jl:1309-1455) ---------------------------------------
+function radial_spectrum(f, g, plan::GBGPlan)
+    n = Ref{Cint}(0); dk = Ref{Cdouble}(0.0)
+    check(ccall((:gbg_radial_shells, libgbg), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cdouble}),
+                plan.handle, n, dk))
+    spectrum = zeros(Float64, n[])
+    check(ccall((:gbg_radial_spectrum, libgbg), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cdouble}, Cint, Ptr{Cvoid}),
+                plan.handle, dptr(f), g === nothing ? C_NULL : dptr(g), ncomp(f), spectrum, n[],
+                C_NULL))
+    return (0:(n[] - 1)) .* dk[], spectrum
+end
+radial_spectrum(f, plan::GBGPlan) = radial_spectrum(f, nothing, plan)
+
+# ---- reality condition (src/Fields.jl:2858-3094) -----------------------------------------
+function reality_residual(f, plan::GBGPlan)
+    r = Ref{Cdouble}(0.0)
+    check(ccall((:gbg_reality_residual, libgbg), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Cdouble}, Ptr{Cvoid}),
+                plan.handle, dptr(f), ncomp(f), r, C_NULL))
+    return r[]
+end
+reality_projection!(dst, src, plan::GBGPlan) =
+    (check(ccall((:gbg_reality_projection, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(src), ncomp(src), C_NULL)); dst)
+reality_projection!(f, plan::GBGPlan) = reality_projection!(f, f, plan)
+
 # ---- the fused right-hand side of examples/HIST_CPU.jl:229-247 --------------------------
 function rhs_uxw!(Nh, uh, plan::GBGPlan, kc)
     E = Ref{Cdouble}(0.0); Z = Ref{Cdouble}(0.0)
