@@ -72,7 +72,8 @@ def test_transforms(G, n, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("n", [(64, 64, 64), (32, 128, 256), (512, 32, 32)])
+@pytest.mark.parametrize("n", [(64, 64, 64), (32, 128, 256), (512, 32, 32), (1024, 16, 16),
+                               (16, 1024, 8), (8, 16, 1024)])
 def test_transforms_generic_vs_fast(G, n, dtype, monkeypatch):
     """Power-of-two lengths take the register-resident passes; the mixed-radix
     LDS passes (GBG_GENERIC_FFT=1 at plan creation) must agree with them and
@@ -191,7 +192,8 @@ def test_operators(G, n, L, dtype):
 
 UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), ((48, 32, 40), None),
              ((64, 64, 128), None), ((128, 128, 32), None), ((64, 64, 64), 5.0), ((64, 64, 64), 1000.0),
-             ((32, 32, 64), 7.3), ((512, 32, 32), 170.5), ((256, 64, 32), 60.0)]
+             ((32, 32, 64), 7.3), ((512, 32, 32), 170.5), ((256, 64, 32), 60.0),
+             ((1024, 32, 32), 340.0), ((32, 1024, 32), 12.0), ((32, 32, 1024), 12.0)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
