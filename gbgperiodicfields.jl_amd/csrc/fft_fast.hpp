@@ -98,7 +98,9 @@ constexpr int slot_step(int c) {
 // Stage (R, LS) of the ping-pong schedule: exchange k (k = 0, 1, ...) writes
 // buffer k mod 2 and the next stage reads it, so one barrier per exchange
 // suffices -- a buffer is rewritten only two exchanges after it was read.
-template <typename T, int N, int E, int LS, int DIR, int SYNC, int W, int PD>
+// ONE (p0 == p1, one buffer where two do not fit in LDS): a block barrier also
+// separates each stage's reads from its writes.
+template <typename T, int N, int E, int LS, int DIR, int SYNC, int W, int PD, bool ONE>
 __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                        const cpx<T>* __restrict__ tw) {
   constexpr int R = StageR<N, LS>::R;
@@ -126,6 +128,7 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
 #pragma unroll
       for (int r = 0; r < R; ++r) a[b][r] = q[r * slot_step<PD>(NR)];
     }
+    if constexpr (ONE && !SYNC && !LAST) __syncthreads();
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -151,20 +154,21 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
       for (int r = 0; r < R; ++r) o[r * (LS == 1 ? 1 : slot_step<PD>(LS))] = a[b][r];
     }
     lds_sync<SYNC>();
-    stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD>(v, p0, p1, t, tw);
+    stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD, ONE>(v, p0, p1, t, tw);
   }
 }
 
 // In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  p0, p1 are
-// this pencil's two LDS regions (slot<PD> layout, at least N elements).  On
-// return p0 may be written at once (its last read was followed by a barrier);
-// p1 may not.
-template <typename T, int N, int E, int DIR, int SYNC, int PD>
+// this pencil's two LDS regions (slot<PD> layout, at least N elements; the
+// same region with ONE).  On return p0 may be written at once (its last read
+// was followed by a barrier); p1 may not.
+template <typename T, int N, int E, int DIR, int SYNC, int PD, bool ONE = false>
 __device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                     const cpx<T>* __restrict__ tw) {
-  stages<T, N, E, 1, DIR, SYNC, 0, PD>(v, p0, p1, t, tw);
-  // with an odd number of exchanges the last one read p0
-  if constexpr ((num_stages(N) - 1) % 2 == 1) lds_sync<SYNC>();
+  stages<T, N, E, 1, DIR, SYNC, 0, PD, ONE>(v, p0, p1, t, tw);
+  // with an odd number of exchanges the last one read p0; with one buffer
+  // the last exchange always read it
+  if constexpr (ONE || (num_stages(N) - 1) % 2 == 1) lds_sync<SYNC>();
 }
 
 
@@ -183,6 +187,7 @@ __device__ __forceinline__ long long xcd_tile() {
 // B pencils per workgroup of the strided passes
 template <typename T, int N>
 struct Cfg {
+  // strided passes: E elements per thread, TP threads per pencil
   static constexpr int E = 8;
   static constexpr int TP = N / E;
   static constexpr int WG = sizeof(T) == 8 ? 512 : 1024;
@@ -193,10 +198,17 @@ struct Cfg {
   // odd pencil stride puts them on distinct banks
   static constexpr int PD = N >= 1024 ? 1 : 0;
   static constexpr int LD = PD ? N + N / 8 + 2 : N + 1;
-  // x pass: row-pair units per workgroup
-  static constexpr int U = TP >= 256 ? 1 : 256 / TP;
-  static constexpr int XNT = U * TP;
-  static constexpr int XSYNC = TP <= 64 ? 1 : 0;
+  // one exchange buffer (and a barrier between each stage's reads and
+  // writes) where two do not fit in the 160 KiB of LDS
+  static constexpr bool ONE = 2 * B * LD * (int)sizeof(cpx<T>) > 152 * 1024;
+  static constexpr int NBUF = ONE ? 1 : 2;
+  // x pass: XE elements per thread (16 for FP32 at N = 1024, so a row sits
+  // in one wave), XTP threads per row, U row-pair units per workgroup
+  static constexpr int XE = (sizeof(T) == 4 && N >= 1024) ? 16 : 8;
+  static constexpr int XTP = N / XE;
+  static constexpr int U = XTP >= 256 ? 1 : 256 / XTP;
+  static constexpr int XNT = U * XTP;
+  static constexpr int XSYNC = XTP <= 64 ? 1 : 0;
   // x pass: consecutive lanes hold consecutive elements of one pencil
   static constexpr int XPD = 1;
   static constexpr int XLD = N + N / 8;
@@ -221,7 +233,7 @@ __device__ __forceinline__ void c2r_pair_fetch(cpx<T> (&a)[E], cpx<T> (&b)[E],
   using C = Cfg<T, N>;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int k = t + C::TP * m;
+    const int k = t + C::XTP * m;
     const int kk = k >= g.nh ? N - k : k;
     a[m] = mk<T>(T(0), T(0));
     b[m] = mk<T>(T(0), T(0));
@@ -238,7 +250,7 @@ __device__ __forceinline__ void c2r_pair_combine(cpx<T> (&v)[E], const cpx<T> (&
   using C = Cfg<T, N>;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int k = t + C::TP * m;
+    const int k = t + C::XTP * m;
     const bool up = k >= g.nh;
     const int kk = up ? N - k : k;
     cpx<T> x = a[m], y = b[m];
@@ -270,9 +282,9 @@ __device__ __forceinline__ void r2c_pair_store(cpx<T> (&v)[E], cpx<T>* pen,
                                                bool oka, bool okb, int t, const G& g) {
   using C = Cfg<T, N>;
 #pragma unroll
-  for (int m = 0; m < E; ++m) pen[t + C::TP * m] = v[m];
+  for (int m = 0; m < E; ++m) pen[t + C::XTP * m] = v[m];
   lds_sync<C::XSYNC>();
-  for (int k = t; k < g.nhk; k += C::TP) {
+  for (int k = t; k < g.nhk; k += C::XTP) {
     const cpx<T> z = pen[k];
     const cpx<T> zc = pen[k == 0 ? 0 : N - k];
     if (oka) ra[k] = mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y));

@@ -127,7 +127,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_fast_c2c(const cpx<T>* in, 
                                                             T scale) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  __shared__ cpx<T> lds[2 * B * LD];
+  __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long p = (long long)blockIdx.x * B + pl;
   const bool valid = p < d.npencil;
@@ -137,7 +137,8 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_fast_c2c(const cpx<T>* in, 
 #pragma unroll
   for (int m = 0; m < E; ++m)
     v[m] = valid ? in[base + (long long)(t + TP * m) * d.se] : mk<T>(T(0), T(0));
-  fast::fft<T, N, E, DIR, 0, C::PD>(v, lds + pl * LD, lds + (B + pl) * LD, t, tw);
+  fast::fft<T, N, E, DIR, 0, C::PD, C::ONE>(v, lds + pl * LD,
+                                            lds + (C::ONE ? pl : B + pl) * LD, t, tw);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < E; ++m) out[base + (long long)(t + TP * m) * d.se] = cscale(v[m], scale);
@@ -155,7 +156,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __rest
                                                                long long nrows, RowG g,
                                                                const cpx<T>* __restrict__ tw) {
   using C = Cfg<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::XLD;
+  constexpr int E = C::XE, TP = C::XTP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_c2r_x(const cpx<T>* _
                                                                RowG g, const cpx<T>* __restrict__ tw,
                                                                T scale) {
   using C = Cfg<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::XLD;
+  constexpr int E = C::XE, TP = C::XTP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
   const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;

@@ -110,7 +110,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ tw, double* __restrict__ part, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  __shared__ cpx<T> lds[2 * B * LD];
+  __shared__ cpx<T> lds[C::NBUF * B * LD];
   __shared__ double red[2][C::NT / 64];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
@@ -145,7 +145,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   for (int m = 0; m < E; ++m) keep[m] = disk && (qxy + tb.ks3[t + TP * m] <= g.hi2);
   double eacc = 0.0, zacc = 0.0;
   cpx<T>* pen = lds + pl * LD;
-  cpx<T>* pen1 = lds + (B + pl) * LD;
+  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   const long long zs = (long long)g.nyc * g.nhp;
   // Each output field reloads the components it needs (the input lives on
   // the 2/3 ball only, so the repeat reads are few and mostly cache hits):
@@ -185,7 +185,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       }
     }
     if (f + 1 < 6) load(f + 1);
-    fast::fft<T, N, E, 1, 0, C::PD>(v, pen, pen1, t, fresh<sizeof(T) == 4>(tw));
+    fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, fresh<sizeof(T) == 4>(tw));
     if (disk) {
       cpx<T>* o = w1 + f * g.s1 + p1;
 #pragma unroll
@@ -252,7 +252,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   constexpr int NF = DIR > 0 ? 6 : 3;
-  __shared__ cpx<T> lds[2 * B * LD];
+  __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
   const long long p2 = tile * B + pl;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   const long long cbase = (long long)z * g.nyc * g.nhp + kx;   // compact ky
   const long long fbase = (long long)z * g.ny * g.nhp + kx;    // all y
   cpx<T>* pen = lds + pl * LD;
-  cpx<T>* pen1 = lds + (B + pl) * LD;
+  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   cpx<T> nxt[E];
   auto load = [&](int f, cpx<T>(&v)[E]) {
     if (DIR > 0) {
@@ -297,7 +297,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = nxt[m];
     if (f + 1 < NF) load(f + 1, nxt);
-    fast::fft<T, N, E, DIR, 0, C::PD>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, DIR, 0, C::PD, C::ONE>(v, pen, pen1, t, tw);
     if (DIR > 0) {
       if (valid) {
         cpx<T>* o = dst + f * g.s2 + fbase;
@@ -332,7 +332,7 @@ template <typename T, int N>
 struct XStage {
   static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
   static constexpr int CH = sizeof(T) == 8 ? 3 : 2;            // copies per field row
-  static constexpr int TPE = Cfg<T, N>::TP * EPL;              // entries per copy per unit
+  static constexpr int TPE = Cfg<T, N>::XTP * EPL;             // entries per copy per unit
   static constexpr int CAP = CH * TPE;                         // staged entries per row
   static constexpr int WAVE_ELEMS = 2 * CH * 64 * EPL;         // staging elements per wave
   static constexpr int ALL8 = (fast::num_stages(N) > 0) && ([] {
@@ -350,7 +350,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
                                                              long long ntiles) {
   using C = Cfg<T, N>;
   using X = XStage<T, N>;
-  constexpr int E = C::E, TP = C::TP, LD = C::XLD, S = C::XSYNC;
+  constexpr int E = C::XE, TP = C::XTP, LD = C::XLD, S = C::XSYNC;
   static_assert(!STG || S, "staging needs a pencil per wave");
   constexpr int NBUF = S ? 1 : 2;
   constexpr int NW = C::XNT / 64;
@@ -484,7 +484,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  __shared__ cpx<T> lds[2 * B * LD];
+  __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long lt = fast::xcd_tile();
   const long long tile = (list && lt < ntiles) ? (long long)list[lt] : lt;
@@ -514,7 +514,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   const long long zs = (long long)g.nyc * g.nhp;
   const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
   cpx<T>* pen = lds + pl * LD;
-  cpx<T>* pen1 = lds + (B + pl) * LD;
+  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   cpx<T> n[3][E];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -523,7 +523,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     for (int m = 0; m < E; ++m) n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0, C::PD>(n[c], pen, pen1, t, tw);
+  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0, C::PD, C::ONE>(n[c], pen, pen1, t, tw);
   if (!valid) return;
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
 #pragma unroll
