@@ -549,19 +549,22 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 }
 
 // The output tiles that hold no kept (kx, ky) pencil: zeros in all kz planes
-// and components, one pencil index per thread, zper kz planes per workgroup row.
+// and components.  Workgroup (x, y) covers kZeroRun consecutive pencil indices
+// of one (component, kz) plane y, so each workgroup streams one contiguous
+// stretch of the output.
+constexpr int kZeroRun = 2048;
 template <typename T>
 __global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
                                                         const int* __restrict__ flag, int b,
-                                                        long long plane, long long M, int nz,
-                                                        int zper) {
-  const long long p5 = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (p5 >= plane || flag[p5 / b]) return;
-  const int z0 = blockIdx.y * zper;
-  const int z1 = z0 + zper < nz ? z0 + zper : nz;
+                                                        long long plane) {
+  cpx<T>* out = nh + (long long)blockIdx.y * plane;
   const cpx<T> zero = mk<T>(T(0), T(0));
-  for (int c = 0; c < 3; ++c)
-    for (int z = z0; z < z1; ++z) nh[c * M + (long long)z * plane + p5] = zero;
+  const long long base = (long long)blockIdx.x * kZeroRun + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < kZeroRun / 256; ++i) {
+    const long long p5 = base + i * 256;
+    if (p5 < plane && !flag[p5 / b]) out[p5] = zero;
+  }
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -751,10 +754,10 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
       return;
     }
     if (ts->nd < ts->nt) {
-      constexpr int zper = 8;
-      const dim3 zg((unsigned)((plane + 255) / 256), (unsigned)((g.nz + zper - 1) / zper));
+      // planes (component c, kz) are consecutive: c M + kz plane = (c nz + kz) plane
+      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun), (unsigned)(3 * g.nz));
       hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, C::B,
-                         plane, g.M, g.nz, zper);
+                         plane);
     }
     if (ts->nd > 0)
       hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
