@@ -476,15 +476,17 @@ def multiply_(dst: Field, a: Field, b: Field, plans: FieldTransformPlans, stream
 # ---- reductions (src/Reductions.jl) -------------------------------------------------
 
 def inner(f: Field, g: Field, plans: FieldTransformPlans, stream=None) -> float:
-    """Spectral ``inner(f, g)`` through Parseval (``src/Reductions.jl:883-925``)."""
-    _need(f, "spectral")
-    _need(g, "spectral")
+    """``inner(f, g)``: spectral through Parseval (``src/Reductions.jl:883-925``)
+    or physical, the cell average of the pointwise contraction (``:838-853``)."""
+    if f.representation != g.representation:
+        raise ArgumentError(1, "inner needs two fields in the same representation")
     _with_plan(f, plans)
     _same_domain(f, g)
     if f.order != g.order:
         raise DimensionMismatch(2, f"sections have different tensor orders: {f.order} and {g.order}")
     out = C.c_double(0.0)
-    check(load().gbg_inner_spectral(plans.handle, f.data, g.data, f.ncomp, C.byref(out), _sh(stream)))
+    fn = load().gbg_inner_spectral if f.spectral else load().gbg_inner_physical
+    check(fn(plans.handle, f.data, g.data, f.ncomp, C.byref(out), _sh(stream)))
     return out.value
 
 

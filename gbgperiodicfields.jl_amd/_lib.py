@@ -65,6 +65,7 @@ SIGNATURES = {
     "gbg_cross": (st, [vp, vp, vp, vp, vp]),
     "gbg_multiply": (st, [vp, vp, vp, vp, vp]),
     "gbg_inner_spectral": (st, [vp, vp, vp, i32, dp, vp]),
+    "gbg_inner_physical": (st, [vp, vp, vp, i32, dp, vp]),
     "gbg_radial_shells": (st, [vp, C.POINTER(C.c_int), dp]),
     "gbg_radial_spectrum": (st, [vp, vp, vp, i32, dp, i32, vp]),
     "gbg_exponential_convolve": (st, [vp, vp, vp, i32, dbl, vp, dbl, dbl, dbl, vp]),

@@ -563,6 +563,22 @@ gbg_status gbg_inner_spectral(gbg_plan* p, const void* f, const void* g, int nco
   GBG_GUARD_END
 }
 
+gbg_status gbg_inner_physical(gbg_plan* p, const void* f, const void* g, int ncomp, double* out,
+                              void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!f || !g || !out) return fail(GBG_E_ARG, "null argument");
+  if (ncomp != 1 && ncomp != 3 && ncomp != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
+  hipStream_t s = as_stream(stream);
+  if ((st = op_inner_physical(p, f, g, ncomp, 2, s))) return st;
+  if ((st = check_hip(hipMemcpyAsync(out, p->red_out + 2, sizeof(double), hipMemcpyDeviceToHost, s),
+                      "inner d2h")))
+    return st;
+  return check_hip(hipStreamSynchronize(s), "inner sync");
+  GBG_GUARD_END
+}
+
 gbg_status gbg_radial_shells(gbg_plan* p, int* nshells, double* dk) {
   GBG_GUARD_BEGIN
   gbg_status st = need_plan(p);

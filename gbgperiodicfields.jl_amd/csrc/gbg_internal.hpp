@@ -174,6 +174,9 @@ gbg_status op_cross(gbg_plan* p, void* dst, const void* u, const void* v, hipStr
 gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hipStream_t s);
 // Parseval inner into p->red_out[slot] (device); no synchronisation
 gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int slot, hipStream_t s);
+// physical inner (sum over grid points / P) into p->red_out[slot]; no synchronisation
+gbg_status op_inner_physical(gbg_plan* p, const void* f, const void* g, int ncomp, int slot,
+                             hipStream_t s);
 
 // exponential kernel, radial spectrum, reality condition (gbg_next.hip)
 gbg_status op_exponential(gbg_plan* p, void* dst, const void* src, int ncomp, double coefficient,

@@ -237,6 +237,26 @@ __global__ __launch_bounds__(kOpNT) void k_inner_part(const cpx<T>* f, const cpx
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// physical inner: sum of a[i] b[i] over every stored real (all components),
+// FP64 partials in the same fixed grid (inner, src/Reductions.jl:838-853)
+template <typename T>
+__global__ __launch_bounds__(kOpNT) void k_dot_part(const T* a, const T* b, size_t total,
+                                                    double* part) {
+  __shared__ double red[kOpNT];
+  const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
+  const size_t lo = chunk * blockIdx.x;
+  const size_t hi = lo + chunk < total ? lo + chunk : total;
+  double acc = 0.0;
+  for (size_t i = lo + threadIdx.x; i < hi; i += kOpNT) acc += (double)a[i] * (double)b[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kOpNT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 __global__ __launch_bounds__(kOpNT) void k_inner_final(const double* part, int nparts, double norm,
                                                        double* out) {
   __shared__ double red[kOpNT];
@@ -399,6 +419,22 @@ gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int sl
   hipLaunchKernelGGL(k_inner_final, 1, kOpNT, 0, s, p->red_part + (size_t)slot * kRedBlocks,
                      kRedBlocks, N * N, p->red_out + slot);
   return check_launch("inner final");
+}
+
+gbg_status op_inner_physical(gbg_plan* p, const void* f, const void* g, int ncomp, int slot,
+                             hipStream_t s) {
+  const size_t total = p->P * (size_t)ncomp;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_dot_part<double>), kRedBlocks, kOpNT, 0, s, (const double*)f,
+                       (const double*)g, total, p->red_part + (size_t)slot * kRedBlocks);
+  else
+    hipLaunchKernelGGL((k_dot_part<float>), kRedBlocks, kOpNT, 0, s, (const float*)f,
+                       (const float*)g, total, p->red_part + (size_t)slot * kRedBlocks);
+  gbg_status st = check_launch("physical inner partials");
+  if (st) return st;
+  hipLaunchKernelGGL(k_inner_final, 1, kOpNT, 0, s, p->red_part + (size_t)slot * kRedBlocks,
+                     kRedBlocks, (double)p->P, p->red_out + slot);
+  return check_launch("physical inner final");
 }
 
 }  // namespace gbg

@@ -127,6 +127,12 @@ gbg_status gbg_multiply(gbg_plan* plan, void* dst, const void* a, const void* b,
 gbg_status gbg_inner_spectral(gbg_plan* plan, const void* f, const void* g, int ncomp, double* out,
                               void* stream);
 
+/* Physical inner (:838-853): sum over every grid point and component of
+ * f g, divided by the grid point count; FP64 accumulation in a fixed order.
+ * Synchronises the stream and writes the host double *out. */
+gbg_status gbg_inner_physical(gbg_plan* plan, const void* f, const void* g, int ncomp, double* out,
+                              void* stream);
+
 /* Radial (shell-binned) spectrum, radial_spectrum(f[, g], m)
  * (src/Reductions.jl:1309-1455): shell m collects the modes with
  * round(|k|/dk) = m (ties to even), dk = 2 pi / max L, Parseval-weighted

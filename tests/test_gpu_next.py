@@ -153,3 +153,28 @@ def test_reality_parity(G, n, dtype):
         assert np.array_equal(df.download(), want)
         G.reality_projection_(df, plans)
         assert np.array_equal(df.download(), want)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(8, 8, 8), (9, 8, 10), (64, 48, 40)])
+def test_inner_physical_parity(G, n, dtype):
+    """Physical inner (src/Reductions.jl:838-853) against the oracle and, by
+    Parseval, against the spectral inner of the transformed fields."""
+    say(f"[inner-physical] n={n} dtype={np.dtype(dtype).name}")
+    g = O.Grid3D(n)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    rng = np.random.default_rng(41)
+    for order in (0, 1):
+        shape = (3,) * order + g.physical_shape
+        f = rng.standard_normal(shape).astype(dtype)
+        h = rng.standard_normal(shape).astype(dtype)
+        df = G.Field(grid, "physical", order, dtype).upload(f)
+        dh = G.Field(grid, "physical", order, dtype).upload(h)
+        got = G.inner(df, dh, plans)
+        ref = O.inner_physical(f.astype(np.float64), h.astype(np.float64), g)
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+        spec = G.inner(G.to_spectral(df, plans), G.to_spectral(dh, plans), plans)
+        assert abs(got - spec) <= TOL[dtype] * max(1.0, abs(got))
+    with pytest.raises(G.ArgumentError):
+        G.inner(df, G.to_spectral(dh, plans), plans)
