@@ -267,6 +267,25 @@ def band_mask(m: Multipliers, k_lo: float, k_hi: float) -> np.ndarray:
     return ((lo2 <= ks) & (ks <= hi2)).astype(np.float64)
 
 
+def mask_digest(m: Multipliers, kc: float) -> tuple[int, str]:
+    """Kept-mode count and SHA-256 of the bit-packed 2/3 mask of
+    ``band_mask(m, 0, kc)`` in storage order (kx fastest), computed one kz
+    plane at a time so full-size grids fit in memory (every plane holds a
+    multiple of 8 modes for the grids this is used on, so the packed planes
+    concatenate to the packed whole)."""
+    import hashlib
+    h = hashlib.sha256()
+    kept = 0
+    hi2 = float(kc) ** 2
+    plane = m.s1[None, :] + m.s2[:, None]          # (Ny, Nh): k1^2 + k2^2
+    for kz in range(m.grid.n[2]):
+        q = plane + m.s3[kz]
+        bits = (0.0 <= q) & (q <= hi2)
+        kept += int(bits.sum())
+        h.update(np.packbits(bits.ravel()).tobytes())
+    return kept, h.hexdigest()
+
+
 def dealias_cutoff(n: int, L: float = TWO_PI) -> float:
     """The consumer's 2/3 cutoff ``(2/3)*(N/2)*(2pi/L)`` evaluated in Float64
     left to right (``examples/HIST_CPU.jl:357``)."""

@@ -51,6 +51,21 @@ def main():
              p_norm=np.array(np.linalg.norm(p)), p_first=p[:2, :3, :4],
              grad_norm=np.array(np.linalg.norm(O.gradient(a, m))))
 
+    # the 2/3 mask at the headline sizes and the batch sizes: kept-mode count
+    # and SHA-256 of the bit-packed mask (storage order)
+    import json
+    shapes = [(64, 64, 64), (512, 512, 512), (1024, 1024, 1024), (32, 32, 32), (48, 48, 48),
+              (96, 96, 96), (128, 128, 128), (128, 128, 256), (256, 64, 64), (256, 256, 256)]
+    out = {}
+    for n in shapes:
+        g = O.Grid3D(n)
+        m = O.Multipliers.build(g)
+        kc = O.dealias_cutoff(min(n))
+        kept, digest = O.mask_digest(m, kc)
+        out["x".join(map(str, n))] = {"kc": kc, "kept": kept, "sha256": digest}
+    with open(os.path.join(HERE, "mask_digests.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
 
 if __name__ == "__main__":
     main()

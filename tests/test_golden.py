@@ -52,3 +52,31 @@ def test_config1_64():
     p = O.dealiased_product(a, b, g, kc, m)
     assert abs(np.linalg.norm(p) - float(d["p_norm"])) < 1e-12 * float(d["p_norm"])
     np.testing.assert_allclose(p[:2, :3, :4], d["p_first"], rtol=0, atol=1e-9)
+
+
+def test_mask_digests():
+    """The 2/3 mask at the headline and batch sizes: the oracle's bit-packed
+    mask digest equals the committed one, and the kept-mode count equals an
+    integer lattice count (n1 >= 0 on the folded axis, |n|^2 <= floor(kc^2)
+    on 2 pi boxes) done without the oracle."""
+    import json
+    import math
+    with open(os.path.join(G, "mask_digests.json")) as fh:
+        digests = json.load(fh)
+    assert len(digests) >= 10
+    for key, want in digests.items():
+        n = tuple(int(v) for v in key.split("x"))
+        g = O.Grid3D(n)
+        m = O.Multipliers.build(g)
+        kc = O.dealias_cutoff(min(n))
+        assert kc == want["kc"]
+        kept, digest = O.mask_digest(m, kc)
+        assert (kept, digest) == (want["kept"], want["sha256"]), key
+        R = math.floor(kc * kc + 1e-9)
+        count = 0
+        for n3 in range(-(n[2] // 2), (n[2] + 1) // 2):
+            for n2 in range(-(n[1] // 2), (n[1] + 1) // 2):
+                r = R - n2 * n2 - n3 * n3
+                if r >= 0:
+                    count += min(n[0] // 2, math.isqrt(r)) + 1
+        assert count == kept, key

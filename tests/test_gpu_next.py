@@ -178,3 +178,31 @@ def test_inner_physical_parity(G, n, dtype):
         assert abs(got - spec) <= TOL[dtype] * max(1.0, abs(got))
     with pytest.raises(G.ArgumentError):
         G.inner(df, G.to_spectral(dh, plans), plans)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("key", ["512x512x512", "256x256x256", "128x128x256", "256x64x64", "48x48x48"])
+def test_mask_digest_full_size(G, key, dtype):
+    """The 2/3 mask the device applies (spectral_cutoff convolve of an all-ones
+    field) at full size is bit-identical to the committed digest."""
+    import hashlib
+    import json
+    import os
+    say(f"[mask] {key} dtype={np.dtype(dtype).name}")
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mask_digests.json")) as fh:
+        want = json.load(fh)[key]
+    n = tuple(int(v) for v in key.split("x"))
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    ones = G.Field(grid, "spectral", 0, dtype).upload(np.ones(grid.spectral_shape(0), dtype=CDT[dtype]))
+    masked = G.convolve_(ones.similar(), ones, G.spectral_cutoff(plans, want["kc"]), plans)
+    got = masked.download()
+    h = hashlib.sha256()
+    kept = 0
+    for kz in range(n[2]):
+        plane = got[kz]
+        bits = plane.real != 0
+        assert np.all((plane.real == 1) | (plane.real == 0)) and np.all(plane.imag == 0)
+        kept += int(bits.sum())
+        h.update(np.packbits(bits.ravel()).tobytes())
+    assert (kept, h.hexdigest()) == (want["kept"], want["sha256"])
