@@ -147,47 +147,57 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   cpx<T>* pen = lds + pl * LD;
   cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   const long long zs = (long long)g.nyc * g.nhp;
-  // Each output field reloads the components it needs (the input lives on
-  // the 2/3 ball only, so the repeat reads are few and mostly cache hits):
-  //   f = 0,1,2: u_f;   f = 3,4,5: w = i (ca u_ia - cb u_ib) with
-  //   w1 = i(a2 u3 - a3 u2), w2 = i(a3 u1 - a1 u3), w3 = i(a1 u2 - a2 u1)
-  // The loads of field f + 1 are issued before the transform of field f.
-  cpx<T> nA[E], nB[E];
-  auto load = [&](int f) {
-    const int ia = f < 3 ? f : (f == 3 ? 2 : (f == 4 ? 0 : 1));
-    const int ib = f == 3 ? 1 : (f == 4 ? 2 : 0);
-    const cpx<T>* ua = uh + ia * g.M + off;
-    const cpx<T>* ub = uh + ib * g.M + off;
+  // Six output fields from four component loads (u1, u3, then u2, then u1
+  // again), each issued at least one transform before it is needed, with two
+  // raw components X, Y held.  Step k = 0, 1, 2 transforms the velocity
+  // component X, then w = i(ca X - cb Y) while the next component loads into
+  // X, then swaps X and Y:
+  //   k = 0: X = u1, Y = u3 -> u1, w2 = i(a3 u1 - a1 u3), load u2
+  //   k = 1: X = u3, Y = u2 -> u3, w1 = i(a2 u3 - a3 u2), load u1
+  //   k = 2: X = u2, Y = u1 -> u2, w3 = i(a1 u2 - a2 u1)
+  // Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
+  cpx<T> X[E], Y[E];
+  auto loadc = [&](cpx<T> (&r)[E], int c) {
+    const cpx<T>* u = uh + c * g.M + off;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const long long zo = (long long)(t + TP * m) * plane;
-      nA[m] = keep[m] ? ua[zo] : mk<T>(T(0), T(0));
-      if (f >= 3) nB[m] = keep[m] ? ub[zo] : mk<T>(T(0), T(0));
-    }
+    for (int m = 0; m < E; ++m)
+      r[m] = keep[m] ? u[(long long)(t + TP * m) * plane] : mk<T>(T(0), T(0));
   };
-  load(0);
+  loadc(X, 0);
+  loadc(Y, 2);
 #pragma unroll 1
-  for (int f = 0; f < 6; ++f) {
+  for (int st = 0; st < 6; ++st) {
+    const int k = st >> 1;
     cpx<T> v[E];
+    int field;
+    if ((st & 1) == 0) {
+      field = k == 0 ? 0 : (k == 1 ? 2 : 1);
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const cpx<T> A = nA[m];
-      if (f < 3) {
-        v[m] = A;
-        eacc += wx * ((double)A.x * A.x + (double)A.y * A.y);
-      } else {
-        const cpx<T> Bv = nB[m];
+      for (int m = 0; m < E; ++m) {
+        v[m] = X[m];
+        eacc += wx * ((double)X[m].x * X[m].x + (double)X[m].y * X[m].y);
+      }
+    } else {
+      field = k == 0 ? 4 : (k == 1 ? 3 : 5);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
         const T a3 = T(tb.kd3[t + TP * m]);
-        const T ca = f == 3 ? a2 : (f == 4 ? a3 : a1);
-        const T cb = f == 3 ? a3 : (f == 4 ? a1 : a2);
-        v[m] = cmuli(csub(cscale(A, ca), cscale(Bv, cb)));
+        const T ca = k == 0 ? a3 : (k == 1 ? a2 : a1);
+        const T cb = k == 0 ? a1 : (k == 1 ? a3 : a2);
+        v[m] = cmuli(csub(cscale(X[m], ca), cscale(Y[m], cb)));
         zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
       }
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const cpx<T> x = X[m];
+        X[m] = Y[m];
+        Y[m] = x;
+      }
+      if (k < 2) loadc(Y, k == 0 ? 1 : 0);
     }
-    if (f + 1 < 6) load(f + 1);
     fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, fresh<sizeof(T) == 4>(tw));
     if (disk) {
-      cpx<T>* o = w1 + f * g.s1 + p1;
+      cpx<T>* o = w1 + field * g.s1 + p1;
 #pragma unroll
       for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
     }
