@@ -90,6 +90,19 @@ __device__ __forceinline__ P* fresh(P* p) {
   return p;
 }
 
+// x where keep holds, else +0, as a bitwise AND with a mask the optimiser
+// cannot trace back to keep.  A select on keep would let it move the load of
+// x under a branch on keep; the load then completes where the branch rejoins
+// instead of staying in flight behind later work.
+template <typename T>
+__device__ __forceinline__ cpx<T> keep_or_zero(cpx<T> x, bool keep) {
+  using U = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+  U msk = keep ? ~U(0) : U(0);
+  asm("" : "+v"(msk));
+  return mk<T>(__builtin_bit_cast(T, __builtin_bit_cast(U, x.x) & msk),
+               __builtin_bit_cast(T, __builtin_bit_cast(U, x.y) & msk));
+}
+
 __device__ __forceinline__ int ky_of(int kyc, const Geo& g) {
   return kyc < g.ya ? kyc : g.ny - g.nyc + kyc;
 }
@@ -136,70 +149,137 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     }
     return;
   }
+  // The transforms' twiddles and the kz derivative factors are read from LDS
+  // copies: a global load consumed inside the loop below would wait for every
+  // load issued before it (the counter of outstanding vector-memory loads
+  // drains in order), including the prefetched components.  The copies are
+  // complete before their first use, which follows the first exchange barrier
+  // of the first transform.  Where LDS cannot hold the twiddle table as well
+  // (FP64, Nz = 1024), it stays in global memory.
+  constexpr bool TWL = (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + 2 * (C::NT / 64) * sizeof(double) +
+                           (size_t)N * (sizeof(cpx<T>) + sizeof(T)) <=
+                       160 * 1024;
+  __shared__ cpx<T> twl[TWL ? N : 1];
+  __shared__ T a3l[N];
+  for (int i = threadIdx.x; i < N; i += C::NT) {
+    if constexpr (TWL) twl[i] = tw[i];
+    a3l[i] = T(tb.kd3[i]);
+  }
+  auto twp = [&]() {
+    if constexpr (TWL)
+      return (const cpx<T>*)twl;
+    else
+      return fresh<true>(tw);
+  };
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
   const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
   const long long plane = (long long)g.ny * g.nh;
   const long long off = (long long)ky * g.nh + kx;
   bool keep[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) keep[m] = disk && (qxy + tb.ks3[t + TP * m] <= g.hi2);
+  for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + tb.ks3[t + TP * m] <= g.hi2);  // no branch
   double eacc = 0.0, zacc = 0.0;
   cpx<T>* pen = lds + pl * LD;
   cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   const long long zs = (long long)g.nyc * g.nhp;
-  // Six output fields from four component loads (u1, u3, then u2, then u1
-  // again), each issued at least one transform before it is needed, with two
-  // raw components X, Y held.  Step k = 0, 1, 2 transforms the velocity
-  // component X, then w = i(ca X - cb Y) while the next component loads into
-  // X, then swaps X and Y:
-  //   k = 0: X = u1, Y = u3 -> u1, w2 = i(a3 u1 - a1 u3), load u2
-  //   k = 1: X = u3, Y = u2 -> u3, w1 = i(a2 u3 - a3 u2), load u1
-  //   k = 2: X = u2, Y = u1 -> u2, w3 = i(a1 u2 - a2 u1)
-  // Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
+  // Six output fields from four component loads, each issued at least one
+  // transform before it is needed, with two raw components X, Y held and no
+  // register copies between them:
+  //   u1 | w2 = i(a3 X - a1 Y), X <- u2 | u3 | w1 = i(a2 Y - a3 X), Y <- u1 | u2 | w3 = i(a1 X - a2 Y)
+  // starting from X = u1, Y = u3.  Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
+  // Entries outside the ball load the pencil's kz = 0 entry instead (a valid
+  // address; on the disk a line the kept lanes fetch anyway) and are zeroed
+  // where they are used.  No branch surrounds a load or a store, so the
+  // compiler's wait for one load never has to cover another issued after it
+  // and each load stays in flight across the transform that follows it.
+  // Every lane of a tile that holds a disk pencil stores (zeros off the disk):
+  // with B dividing the row pitch nhp, all of them address the buffer.
+  static_assert(16 % B == 0 || B > 16, "tile width");
+  constexpr bool ALL_STORE = 16 % B == 0;
+  const bool st_ok = ALL_STORE || p1 < (long long)g.nyc * g.nhp;
   cpx<T> X[E], Y[E];
   auto loadc = [&](cpx<T> (&r)[E], int c) {
     const cpx<T>* u = uh + c * g.M + off;
 #pragma unroll
-    for (int m = 0; m < E; ++m)
-      r[m] = keep[m] ? u[(long long)(t + TP * m) * plane] : mk<T>(T(0), T(0));
-  };
-  loadc(X, 0);
-  loadc(Y, 2);
-#pragma unroll 1
-  for (int st = 0; st < 6; ++st) {
-    const int k = st >> 1;
-    cpx<T> v[E];
-    int field;
-    if ((st & 1) == 0) {
-      field = k == 0 ? 0 : (k == 1 ? 2 : 1);
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        v[m] = X[m];
-        eacc += wx * ((double)X[m].x * X[m].x + (double)X[m].y * X[m].y);
-      }
-    } else {
-      field = k == 0 ? 4 : (k == 1 ? 3 : 5);
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const T a3 = T(tb.kd3[t + TP * m]);
-        const T ca = k == 0 ? a3 : (k == 1 ? a2 : a1);
-        const T cb = k == 0 ? a1 : (k == 1 ? a3 : a2);
-        v[m] = cmuli(csub(cscale(X[m], ca), cscale(Y[m], cb)));
-        zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
-      }
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const cpx<T> x = X[m];
-        X[m] = Y[m];
-        Y[m] = x;
-      }
-      if (k < 2) loadc(Y, k == 0 ? 1 : 0);
+    for (int m = 0; m < E; ++m) {
+      long long zi = keep[m] ? (long long)(t + TP * m) * plane : 0;
+      asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+      r[m] = u[zi];
     }
-    fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, fresh<sizeof(T) == 4>(tw));
-    if (disk) {
+  };
+  auto emit = [&](cpx<T> (&v)[E], int field) {
+    fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
+    if (st_ok) {
       cpx<T>* o = w1 + field * g.s1 + p1;
 #pragma unroll
       for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto velocity = [&](const cpx<T> (&A)[E], int field) {
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      v[m] = keep_or_zero(A[m], keep[m]);
+      eacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+    }
+    emit(v, field);
+  };
+  // w = i(ca A - cb B), ia / ib = 1, 2, 3 naming which of a1, a2, a3 ca and cb are;
+  // then the next component (nc >= 0) loads into A
+  auto vorticity = [&](cpx<T> (&A)[E], const cpx<T> (&Bv)[E], int ia, int ib, int field, int nc) {
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const T a3 = a3l[t + TP * m];
+      const T ca = ia == 1 ? a1 : (ia == 2 ? a2 : a3);
+      const T cb = ib == 1 ? a1 : (ib == 2 ? a2 : a3);
+      v[m] = keep_or_zero(cmuli(csub(cscale(A[m], ca), cscale(Bv[m], cb))), keep[m]);
+      zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+    }
+    if (nc >= 0) loadc(A, nc);
+    emit(v, field);
+  };
+  // The six transforms written out one after another hold more registers
+  // than a loop over them; where that spills (short pencils, where more
+  // pencils share a workgroup), a loop over the fields loads the components
+  // of field f + 1 (one or two, nine in all) while field f transforms.
+  constexpr bool UNROLLED = sizeof(T) == 8 ? N >= 256 : N >= 512;
+  if constexpr (UNROLLED) {
+    loadc(X, 0);
+    loadc(Y, 2);
+    velocity(X, 0);
+    vorticity(X, Y, 3, 1, 4, 1);
+    velocity(Y, 2);
+    vorticity(Y, X, 2, 3, 3, 0);
+    velocity(X, 1);
+    vorticity(X, Y, 1, 2, 5, -1);
+  } else {
+    // field f: f = 0, 1, 2 -> u_f;  f = 3, 4, 5 -> w = i (ca u_ia - cb u_ib)
+    auto load = [&](int f) {
+      loadc(X, f < 3 ? f : (f == 3 ? 2 : (f == 4 ? 0 : 1)));
+      if (f >= 3) loadc(Y, f == 3 ? 1 : (f == 4 ? 2 : 0));
+    };
+    load(0);
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      cpx<T> v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        if (f < 3) {
+          v[m] = keep_or_zero(X[m], keep[m]);
+          eacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+        } else {
+          const T a3 = a3l[t + TP * m];
+          const T ca = f == 3 ? a2 : (f == 4 ? a3 : a1);
+          const T cb = f == 3 ? a3 : (f == 4 ? a1 : a2);
+          v[m] = keep_or_zero(cmuli(csub(cscale(X[m], ca), cscale(Y[m], cb))), keep[m]);
+          zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+        }
+      }
+      if (f + 1 < 6) load(f + 1);
+      emit(v, f);
     }
   }
   // FP64 block reduction of the energy / enstrophy partial sums, fixed order
@@ -262,17 +342,43 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   constexpr int NF = DIR > 0 ? 6 : 3;
+  // Two schedules, chosen by measurement (MI355X, 512^3 and 1024^3, FP64 and
+  // FP32):
+  //  STREAM (inverse pass, N <= 512): no branch around any load or store and
+  //    the field loop written out, so the compiler's waits for the field in
+  //    hand never include the prefetched next field or earlier stores
+  //    (outstanding loads and stores retire from one in-order counter); the
+  //    twiddles come from an LDS copy for the same reason.
+  //  otherwise: a loop over the fields, loads and stores under their masks,
+  //    twiddles from global memory (at N = 1024 the written-out form and the
+  //    LDS or opaque twiddle reads each cost about a quarter more time).
+  constexpr bool STREAM = DIR > 0 && N <= 512;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
+  if (tile >= ntiles) return;
   const long long p2 = tile * B + pl;
-  int z = 0, kx = 0;
+  constexpr bool TWL =
+      STREAM && (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + (size_t)N * sizeof(cpx<T>) <= 160 * 1024;
+  __shared__ cpx<T> twl[TWL ? N : 1];
+  // complete before its first use, which follows the first exchange barrier
+  if constexpr (TWL)
+    for (int i = threadIdx.x; i < N; i += C::NT) twl[i] = tw[i];
+  auto twp = [&]() {
+    if constexpr (TWL)
+      return (const cpx<T>*)twl;
+    else
+      return tw;
+  };
+  // col: the lane's column in the row pitch nhp (its padding column when
+  // col >= nhk), used for addresses; kx: the same, 0 in padding, for tables
+  int z = 0, col = 0, kx = 0;
   bool valid = false;
-  if (tile < ntiles && p2 < (long long)g.nz * g.nhp) {
+  if (p2 < (long long)g.nz * g.nhp) {
     z = (int)(p2 / g.nhp);
-    kx = (int)(p2 - (long long)z * g.nhp);
-    valid = kx < g.nhk;
-    if (!valid) kx = 0;
+    col = (int)(p2 - (long long)z * g.nhp);
+    valid = col < g.nhk;
+    kx = valid ? col : 0;
   }
   const double q1 = tb.ks1[kx];
   // compact (disk) offsets of this thread's elements; -1 = not held
@@ -283,33 +389,42 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
     const int kyc = kyc_of(ky, g);
     coff[m] = (valid && kyc >= 0 && q1 + tb.ks2[ky] <= g.hi2) ? kyc * g.nhp : -1;
   }
-  const long long cbase = (long long)z * g.nyc * g.nhp + kx;   // compact ky
-  const long long fbase = (long long)z * g.ny * g.nhp + kx;    // all y
+  const long long cbase = (long long)z * g.nyc * g.nhp + col;   // compact ky
+  const long long fbase = (long long)z * g.ny * g.nhp + col;    // all y
   cpx<T>* pen = lds + pl * LD;
   cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
+  auto held = [&](int m) { return DIR > 0 ? coff[m] >= 0 : valid; };
+  auto idx = [&](int m) {
+    return DIR > 0 ? (long long)coff[m] : (long long)(t + TP * m) * g.nhp;
+  };
+  // STREAM: entries not held read the pencil's first entry (a valid address)
+  // and are zeroed where used.  With B dividing the row pitch every lane of
+  // a launched tile addresses the buffer, and all of them store (zeros from
+  // the padding columns kx >= nhk).
+  constexpr bool ALL_STORE = 16 % B == 0;
+  const bool st_ok = STREAM ? (ALL_STORE || p2 < (long long)g.nz * g.nhp) : valid;
   cpx<T> nxt[E];
   auto load = [&](int f, cpx<T>(&v)[E]) {
-    if (DIR > 0) {
-      const cpx<T>* in = src + f * g.s1 + cbase;
+    const cpx<T>* in = DIR > 0 ? src + f * g.s1 + cbase : src + f * g.s2 + fbase;
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = coff[m] >= 0 ? in[coff[m]] : mk<T>(T(0), T(0));
-    } else {
-      const cpx<T>* in = src + f * g.s2 + fbase;
-#pragma unroll
-      for (int m = 0; m < E; ++m)
-        v[m] = valid ? in[(long long)(t + TP * m) * g.nhp] : mk<T>(T(0), T(0));
+    for (int m = 0; m < E; ++m) {
+      if constexpr (STREAM) {
+        long long zi = held(m) ? idx(m) : 0;
+        asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+        v[m] = in[zi];
+      } else {
+        v[m] = held(m) ? in[idx(m)] : mk<T>(T(0), T(0));
+      }
     }
   };
-  load(0, nxt);
-#pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
+  auto step = [&](int f) {
     cpx<T> v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = nxt[m];
+    for (int m = 0; m < E; ++m) v[m] = STREAM ? keep_or_zero(nxt[m], held(m)) : nxt[m];
     if (f + 1 < NF) load(f + 1, nxt);
-    fast::fft<T, N, E, DIR, 0, C::PD, C::ONE>(v, pen, pen1, t, tw);
+    fast::fft<T, N, E, DIR, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
     if (DIR > 0) {
-      if (valid) {
+      if (st_ok) {
         cpx<T>* o = dst + f * g.s2 + fbase;
 #pragma unroll
         for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * g.nhp] = v[m];
@@ -320,6 +435,18 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
       for (int m = 0; m < E; ++m)
         if (coff[m] >= 0) o[coff[m]] = v[m];
     }
+  };
+  load(0, nxt);
+  if constexpr (STREAM) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      step(f);
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll 1
+    for (int f = 0; f < NF; ++f) step(f);
   }
 }
 
@@ -525,22 +652,62 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
   cpx<T>* pen = lds + pl * LD;
   cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
+  // A global load consumed between the component loads and their last use
+  // would wait for all of them (the counter of outstanding loads drains in
+  // order): for N <= 512 (STREAM) the twiddles come from an LDS copy
+  // (complete before the first exchange barrier) and the component loads
+  // carry no branch; at N = 1024 the plain global reads measured faster.  The
+  // mask bits are computed before the components load.
+  constexpr bool STREAM = N <= 512;
+  constexpr bool TWL =
+      STREAM && (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + (size_t)N * sizeof(cpx<T>) <= 160 * 1024;
+  __shared__ cpx<T> twl[TWL ? N : 1];
+  if constexpr (TWL)
+    for (int i = threadIdx.x; i < N; i += C::NT) twl[i] = tw[i];
+  auto twp = [&]() {
+    if constexpr (TWL)
+      return (const cpx<T>*)twl;
+    else
+      return tw;
+  };
+  bool keep[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int kz = t + TP * m;
+    keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) & !(kx == 0 && ky == 0 && kz == 0);  // no branch
+  }
+  // STREAM: components off the disk read the compact buffer's first entry (a
+  // valid address) and are zeroed just before their transform
   cpx<T> n[3][E];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const cpx<T>* in = w1 + c * g.s1 + cb;
 #pragma unroll
-    for (int m = 0; m < E; ++m) n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
+    for (int m = 0; m < E; ++m) {
+      if constexpr (STREAM) {
+        long long zi = disk ? (long long)(t + TP * m) * zs : 0;
+        asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+        n[c][m] = in[zi];
+      } else {
+        n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
+      }
+    }
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) fast::fft<T, N, E, -1, 0, C::PD, C::ONE>(n[c], pen, pen1, t, tw);
+  for (int c = 0; c < 3; ++c) {
+    if constexpr (STREAM) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) n[c][m] = keep_or_zero(n[c][m], disk);
+    }
+    fast::fft<T, N, E, -1, 0, C::PD, C::ONE>(n[c], pen, pen1, t, twp());
+    if constexpr (STREAM) __builtin_amdgcn_sched_barrier(0);
+  }
   if (!valid) return;
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int kz = t + TP * m;
     const T a3 = T(tb.kd3[kz]);
-    const bool keep = disk && (qxy + tb.ks3[kz] <= g.hi2) && !(kx == 0 && ky == 0 && kz == 0);
     // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
     const T sq = a1 * a1 + a2 * a2 + a3 * a3;
     const T den = sq == T(0) ? T(1) : sq;
@@ -552,9 +719,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const long long o = (long long)kz * plane + p5;
     // component-wise selects (a select of whole complex values is lowered
     // through a private array in FP64)
-    nh[o] = mk<T>(keep ? p1.x : T(0), keep ? p1.y : T(0));
-    nh[g.M + o] = mk<T>(keep ? p2.x : T(0), keep ? p2.y : T(0));
-    nh[2 * g.M + o] = mk<T>(keep ? p3.x : T(0), keep ? p3.y : T(0));
+    nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
+    nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
+    nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
   }
 }
 
