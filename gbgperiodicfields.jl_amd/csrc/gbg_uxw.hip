@@ -550,16 +550,29 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
     }
   };
   T N3[E], n3a[E];
+  // EARLY (N <= 512): the raw entries na, nb of the second row's first field
+  // pair are fetched before the first row's output stores are issued -- the
+  // staging wait covers every outstanding load and store, so fetching them
+  // after the stores would also wait for the stores to complete -- and the
+  // two rows are written out.  Where that form spills (FP64 at N = 32, 256
+  // and 1024, FP32 at 1024), the rows loop and each fetches its first pair.
+  constexpr bool EARLY = sizeof(T) == 4 ? N <= 512 : (N == 64 || N == 128 || N == 512);
+  cpx<T> na[E], nb[E];
   issue(ro0, 0);
-#pragma unroll 1
-  for (int h = 0; h < 2; ++h) {
+  if constexpr (EARLY) {
+    fetch(na, nb, ro0, ok0, 0);
+    issue(ro0, 1);
+  }
+  auto row = [&](int h) {
     const long long ro = h ? ro1 : ro0;
     const bool ok = h ? ok1 : ok0;
     T U1[E], W1[E], U2[E], W2[E];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       cpx<T> z[E];
-      {
+      if (EARLY && q == 0) {
+        fast::c2r_pair_combine<T, N, E>(z, na, nb, t, g);
+      } else {
         cpx<T> a[E], b[E];
         fetch(a, b, ro, ok, q);
         if (q < 2)
@@ -589,6 +602,10 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
           z[m] = mk<T>(U2[m] * w3v - u3 * W2[m], u3 * W1[m] - U1[m] * w3v);
         }
         fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, twp());
+        if (EARLY && h == 0) {
+          fetch(na, nb, ro1, ok1, 0);
+          issue(ro1, 1);
+        }
         fast::r2c_pair_store<T, N, E>(z, pen, w3 + ro, w3 + g.s2 + ro, ok, ok, t, g);
       }
     }
@@ -601,6 +618,13 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
           n3a[m] = N3[m];
       }
     }
+  };
+  if constexpr (EARLY) {
+    row(0);
+    row(1);
+  } else {
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) row(h);
   }
   cpx<T> v[E];
 #pragma unroll
