@@ -269,6 +269,91 @@ def run_ops(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
     return res
 
 
+def run_rows(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
+    """Each operator of the spectral path on its own at --size, timed by HIP
+    events on the launch stream, with its algorithmic bytes (the fused-pass
+    ideal: every pass reads and writes each participating component once;
+    spectral component = M complex elements, physical component = P reals)
+    and the fraction of the 8 TB/s HBM peak that implies."""
+    from gbg_amd import workload
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    es = np.dtype(dtype).itemsize
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    P = n[0] * n[1] * n[2]
+    sc, ph = M * 2 * es, P * es            # one spectral / one physical component
+    u = workload.synthetic_field(n, 2001, vector=True)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dv = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dsrc = G.Field(grid, "spectral", 1, dtype).upload(u)     # consumed by the 3-arg inverse
+    da = G.Field(grid, "spectral", 0, dtype).upload(u[0])
+    db = G.Field(grid, "spectral", 0, dtype).upload(u[1])
+    del u
+    ds = G.Field(grid, "spectral", 0, dtype)
+    dt = G.Field(grid, "spectral", 2, dtype)
+    pu = G.to_physical(du, plans, stream=stream)
+    pv = G.to_physical(dv, plans, stream=stream)
+    pw = G.Field(grid, "physical", 1, dtype)
+    pa = G.to_physical(da, plans, stream=stream)
+    pb = G.to_physical(db, plans, stream=stream)
+    pc = G.Field(grid, "physical", 0, dtype)
+    band = G.spectral_cutoff(plans, G.dealias_cutoff(n[0]))
+    stream.synchronize()
+    cases = [
+        ("to_spectral (vector)", 3 * (ph + 5 * sc), lambda: G.to_spectral_(dv, pu, plans, stream=stream)),
+        ("to_physical 3-arg (vector)", 3 * (ph + 5 * sc),
+         lambda: G.to_physical_(pw, dsrc, plans, stream=stream)),
+        ("gradient scalar->vector", 4 * sc, lambda: G.gradient_(dv, da, plans, stream=stream)),
+        ("gradient vector->tensor", 12 * sc, lambda: G.gradient_(dt, du, plans, stream=stream)),
+        ("divergence vector->scalar", 4 * sc, lambda: G.divergence_(ds, du, plans, stream=stream)),
+        ("curl", 6 * sc, lambda: G.curl_(dv, du, plans, stream=stream)),
+        ("inverse_curl", 6 * sc, lambda: G.inverse_curl_(dv, du, plans, stream=stream)),
+        ("laplacian (vector)", 6 * sc, lambda: G.laplacian_(dv, du, plans, stream=stream)),
+        ("inverse_laplacian (scalar)", 2 * sc, lambda: G.inverse_laplacian_(ds, da, plans, stream=stream)),
+        ("helmholtz_projection", 6 * sc, lambda: G.helmholtz_projection_(dv, du, plans, stream=stream)),
+        ("convolve 2/3 cutoff (vector)", 6 * sc, lambda: G.convolve_(dv, du, band, plans, stream=stream)),
+        ("cross (physical)", 9 * ph, lambda: G.cross_(pw, pu, pv, plans, stream=stream)),
+        ("multiply (physical scalar)", 3 * ph, lambda: G.multiply_(pc, pa, pb, plans, stream=stream)),
+        ("mean_free_projection (vector)", None, lambda: G.mean_free_projection_(dv, plans, stream=stream)),
+        ("dealiased_product (scalar)", 15 * sc,
+         lambda: G.dealiased_product_(ds, da, db, plans, stream=stream)),
+    ]
+    res = {}
+    for name, nbytes, call in cases:
+        for _ in range(warmup):
+            call()
+        stream.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream.handle)
+        for _ in range(steps):
+            call()
+        ev.record(e1, stream.handle)
+        ms = ev.elapsed(e0, e1) / steps
+        rec = {"ms": round(ms, 4)}
+        if nbytes:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            rec.update({"bytes_alg": int(nbytes), "achieved_GBs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        res[name] = rec
+        print(f"[rows] {name}: {rec}", file=sys.stderr, flush=True)
+    # the reductions return a host scalar (a device-to-host copy and a stream
+    # synchronisation per call): timed on the host clock
+    for name, nbytes, call in [
+            ("inner spectral (vector)", 6 * sc, lambda: G.inner(du, dv, plans, stream=stream)),
+            ("inner physical (vector)", 6 * ph, lambda: G.inner(pu, pv, plans, stream=stream))]:
+        for _ in range(warmup):
+            call()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "bytes_alg": int(nbytes), "achieved_GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "clock": "host, includes the result copy"}
+        print(f"[rows] {name}: {res[name]}", file=sys.stderr, flush=True)
+    return res
+
+
 def smooth_sizes(lo=32, hi=512):
     out = []
     for n in range(lo, hi + 1, 2):
@@ -408,11 +493,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--form", default="uxw", choices=["uxw", "batch", "serve", "hbm", "ops"],
+    ap.add_argument("--form", default="uxw",
+                    choices=["uxw", "batch", "serve", "hbm", "ops", "rows"],
                     help="uxw: the headline 512^3 term (default); batch: the 96-field batched "
                          "graph form; serve: the Poisson request stream; hbm: the streaming "
                          "copy/fill rates of the device; ops: exponential kernel, radial "
-                         "spectrum and reality projection at --size")
+                         "spectrum and reality projection at --size; rows: every transform, "
+                         "operator, product and reduction on its own at --size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device twice (a plain copy kernel of known "
                          "size, the scale for the HBM byte counters)")
@@ -460,6 +547,22 @@ def main():
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"],
                             "traffic": None},
                "ops": res}
+        print(json.dumps(out), flush=True)
+        return
+    if args.form == "rows":
+        res = run_rows(G, stream, ev, args.steps, args.warmup, n, dtype)
+        first = res["to_spectral (vector)"]
+        out = {"metric": f"{n[0]}^3 spectral-path operators on their own: GB/s by algorithmic "
+                         "bytes (headline: to_spectral of a vector field)",
+               "value": first["achieved_GBs"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": first["ms"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+               "data": "synthetic (seeded k^-5/3 solenoidal spectrum, seed 2001)",
+               "config": {"workload": f"{n[0]}x{n[1]}x{n[2]} fields, one operator per line"},
+               "roofline": {"bound": "hbm", "achieved": first["achieved_GBs"],
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"],
+                            "traffic": None},
+               "rows": res}
         print(json.dumps(out), flush=True)
         return
     if args.form == "hbm":

@@ -203,5 +203,7 @@ gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, c
 // work-area bytes pipe_uxw needs on this plan's grid for cutoff kc (reads
 // only the plan's n, L, nh, M, P, elem and generic_only)
 size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags);
+// work-area bytes pipe_dealiased_product needs (same fields read as above)
+size_t dealiased_product_work_bytes(gbg_plan* p, double kc);
 
 }  // namespace gbg

@@ -101,12 +101,12 @@ static void on_done(void* p) {
 static size_t work_need(const gbg_job& j) {
   const size_t M = (size_t)(j.n[0] / 2 + 1) * j.n[1] * j.n[2];
   const size_t P = (size_t)j.n[0] * j.n[1] * j.n[2];
-  const size_t cs = 2 * (size_t)j.elem_bytes, rs = j.elem_bytes;
+  const size_t cs = 2 * (size_t)j.elem_bytes;
   switch (j.kind) {
     case GBG_JOB_TO_PHYSICAL: return (size_t)j.ncomp * M * cs;
-    case GBG_JOB_DEALIASED_PRODUCT: return 2 * M * cs + 2 * P * rs;
+    case GBG_JOB_DEALIASED_PRODUCT:
     case GBG_JOB_UXW: {
-      // the fused path's compact intermediates depend on the cutoff
+      // the fused paths' compact intermediates depend on the cutoff
       gbg_plan geo;
       std::memset(&geo, 0, sizeof(geo));
       for (int a = 0; a < 3; ++a) {
@@ -119,7 +119,8 @@ static size_t work_need(const gbg_job& j) {
       geo.P = P;
       const char* env = std::getenv("GBG_GENERIC_FFT");
       geo.generic_only = (env && env[0] == '1') ? 1 : 0;
-      return uxw_work_bytes(&geo, j.kc, 0);
+      return j.kind == GBG_JOB_UXW ? uxw_work_bytes(&geo, j.kc, 0)
+                                   : dealiased_product_work_bytes(&geo, j.kc);
     }
     default: return 0;
   }

@@ -328,20 +328,94 @@ __global__ __launch_bounds__(256) void k_uxw_reduce(const double* __restrict__ p
   }
 }
 
+// ---- P1: z-inverse of two scalar spectra (the dealiased product) -------------
+// The tiles of K1; fields 0, 1 of the compact buffer receive a^ and b^ on the
+// disk pencils, with the modes outside the 2/3 ball zeroed on load.  b's
+// loads are in flight while a transforms; as in K1 no load or store sits
+// under a branch and the twiddles are read from an LDS copy where it fits.
+template <typename T, int N>
+__global__ __launch_bounds__((Cfg<T, N>::NT)) void k_dp_z_inverse(
+    const cpx<T>* __restrict__ ah, const cpx<T>* __restrict__ bh, cpx<T>* __restrict__ w1, Geo g,
+    Tables tb, const cpx<T>* __restrict__ tw, long long ntiles) {
+  using C = Cfg<T, N>;
+  constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
+  __shared__ cpx<T> lds[C::NBUF * B * LD];
+  const int pl = threadIdx.x % B, t = threadIdx.x / B;
+  const long long tile = fast::xcd_tile();
+  const long long p1 = tile * B + pl;
+  int kx = 0, ky = 0;
+  double qxy = 0.0;
+  bool disk = false;
+  if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
+    const int kyc = (int)(p1 / g.nhp);
+    kx = (int)(p1 - (long long)kyc * g.nhp);
+    if (kx < g.nhk) {
+      ky = ky_of(kyc, g);
+      qxy = tb.ks1[kx] + tb.ks2[ky];
+      disk = qxy <= g.hi2;
+    } else {
+      kx = 0;
+    }
+  }
+  if (!__syncthreads_or(disk)) return;
+  constexpr bool TWL =
+      (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + (size_t)N * sizeof(cpx<T>) <= 160 * 1024;
+  __shared__ cpx<T> twl[TWL ? N : 1];
+  if constexpr (TWL)
+    for (int i = threadIdx.x; i < N; i += C::NT) twl[i] = tw[i];
+  auto twp = [&]() {
+    if constexpr (TWL)
+      return (const cpx<T>*)twl;
+    else
+      return tw;
+  };
+  const long long plane = (long long)g.ny * g.nh;
+  const long long off = (long long)ky * g.nh + kx;
+  const long long zs = (long long)g.nyc * g.nhp;
+  bool keep[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + tb.ks3[t + TP * m] <= g.hi2);  // no branch
+  cpx<T>* pen = lds + pl * LD;
+  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
+  constexpr bool ALL_STORE = 16 % B == 0;   // every lane of a launched tile addresses w1
+  const bool st_ok = ALL_STORE || p1 < (long long)g.nyc * g.nhp;
+  cpx<T> A[E], Bv[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    long long zi = keep[m] ? (long long)(t + TP * m) * plane : 0;
+    asm("" : "+v"(zi));  // keeps the optimiser from splitting the loads by cases
+    A[m] = ah[off + zi];
+    Bv[m] = bh[off + zi];
+  }
+  auto emit = [&](cpx<T> (&v)[E], int field) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = keep_or_zero(v[m], keep[m]);
+    fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
+    if (st_ok) {
+      cpx<T>* o = w1 + field * g.s1 + p1;
+#pragma unroll
+      for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
+    }
+  };
+  emit(A, 0);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+  emit(Bv, 1);
+}
+
 // ---- K2 / K4: y passes over (z, kx < nhk) pencils ------------------------------
-// DIR = +1 (inverse, 6 fields): loads the disk ky from the compact buffer,
-//   stores all y to the full-y buffer.
-// DIR = -1 (forward, 3 fields): loads all y from the full-y buffer, stores
-//   the disk ky to the compact buffer.
+// DIR = +1 (inverse; NF = 6 for u x omega, 2 for the scalar product): loads
+//   the disk ky from the compact buffer, stores all y to the full-y buffer.
+// DIR = -1 (forward; NF = 3, or 1): loads all y from the full-y buffer,
+//   stores the disk ky to the compact buffer.
 // The next field's loads are issued before the current field's transform.
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, int NF>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restrict__ src,
                                                          cpx<T>* __restrict__ dst, Geo g,
                                                          Tables tb, const cpx<T>* __restrict__ tw,
                                                          long long ntiles) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
-  constexpr int NF = DIR > 0 ? 6 : 3;
   // Two schedules, chosen by measurement (MI355X, 512^3 and 1024^3, FP64 and
   // FP32):
   //  STREAM (inverse pass, N <= 512): no branch around any load or store and
@@ -465,6 +539,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 // LDS staging area (global_load_lds) while the current pair is transformed.
 // Without S, FP64 table loads go through fresh() so the optimiser does not
 // hoist the twiddles of every stage and hold them in registers.
+// PROD (the dealiased scalar product): the two fields a, b of a row share one
+// complex transform, p = a b is formed in registers, and p of the two rows of
+// the unit share one forward transform (one output field).
 template <typename T, int N>
 struct XStage {
   static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
@@ -480,7 +557,7 @@ struct XStage {
   static constexpr int TWN = ALL8 ? N / 2 : N;                 // twiddle entries the stages read
 };
 
-template <typename T, int N, bool STG>
+template <typename T, int N, bool STG, bool PROD>
 __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
                                                              cpx<T>* __restrict__ w3, Geo g,
                                                              const cpx<T>* __restrict__ tw, T scale,
@@ -488,6 +565,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
   using C = Cfg<T, N>;
   using X = XStage<T, N>;
   constexpr int E = C::XE, TP = C::XTP, LD = C::XLD, S = C::XSYNC;
+  constexpr int QOFF = PROD ? 1 : 3;   // field pairs (q, q + QOFF)
   static_assert(!STG || S, "staging needs a pencil per wave");
   constexpr int NBUF = S ? 1 : 2;
   constexpr int NW = C::XNT / 64;
@@ -523,7 +601,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
             (const void __attribute__((address_space(1)))*)(w2 + q * g.s2 + ro + e),
             (void __attribute__((address_space(3)))*)(sw + c * 64 * X::EPL), 16, 0, 0);
         __builtin_amdgcn_global_load_lds(
-            (const void __attribute__((address_space(1)))*)(w2 + (q + 3) * g.s2 + ro + e),
+            (const void __attribute__((address_space(1)))*)(w2 + (q + QOFF) * g.s2 + ro + e),
             (void __attribute__((address_space(3)))*)(sw + (X::CH + c) * 64 * X::EPL), 16, 0, 0);
       }
     }
@@ -546,9 +624,37 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
       }
       __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0): staging reads done before reuse
     } else {
-      fast::c2r_pair_fetch<T, N, E>(a, b, w2 + q * g.s2 + ro, w2 + (q + 3) * g.s2 + ro, ok, ok, t, g);
+      fast::c2r_pair_fetch<T, N, E>(a, b, w2 + q * g.s2 + ro, w2 + (q + QOFF) * g.s2 + ro, ok, ok,
+                                    t, g);
     }
   };
+  if constexpr (PROD) {
+    // row 0: fetch (a, b), stage row 1, transform, p0 = a b; row 1 the same;
+    // then p0 + i p1 forward and split into the two rows' half spectra
+    T P0[E];
+    cpx<T> z[E];
+    issue(ro0, 0);
+    {
+      cpx<T> a[E], b[E];
+      fetch(a, b, ro0, ok0, 0);
+      issue(ro1, 0);
+      fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
+    }
+    fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
+#pragma unroll
+    for (int m = 0; m < E; ++m) P0[m] = (z[m].x * scale) * (z[m].y * scale);
+    {
+      cpx<T> a[E], b[E];
+      fetch(a, b, ro1, ok1, 0);
+      fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
+    }
+    fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[m] = mk<T>(P0[m], (z[m].x * scale) * (z[m].y * scale));
+    fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, twp());
+    fast::r2c_pair_store<T, N, E>(z, pen, w3 + ro0, w3 + ro1, ok0, ok1, t, g);
+    return;
+  }
   T N3[E], n3a[E];
   // EARLY (N <= 512): the raw entries na, nb of the second row's first field
   // pair are fetched before the first row's output stores are issued -- the
@@ -639,7 +745,9 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
 // list, workgroup i takes tile list[i] (the tiles that hold kept pencils) and
 // k_uxw_zero_tiles writes the others; without one, workgroup i takes tile i
 // and tiles without a kept pencil are written as zeros here.
-template <typename T, int N>
+// NC = 3: u x omega (Leray projection, mask, mean-free); NC = 1: the scalar
+// product (mask only: its mean mode is kept).
+template <typename T, int N, int NC>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ w1, cpx<T>* __restrict__ nh, Geo g, Tables tb,
     const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles) {
@@ -666,7 +774,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   if (!__syncthreads_or(disk)) {
     if (valid) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int m = 0; m < E; ++m) nh[c * g.M + (long long)(t + TP * m) * plane + p5] = mk<T>(T(0), T(0));
     }
@@ -698,13 +806,14 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int kz = t + TP * m;
-    keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) & !(kx == 0 && ky == 0 && kz == 0);  // no branch
+    keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) &  // no branch
+              (NC == 1 || !(kx == 0 && ky == 0 && kz == 0));
   }
   // STREAM: components off the disk read the compact buffer's first entry (a
   // valid address) and are zeroed just before their transform
-  cpx<T> n[3][E];
+  cpx<T> n[NC][E];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < NC; ++c) {
     const cpx<T>* in = w1 + c * g.s1 + cb;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -718,7 +827,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     }
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < NC; ++c) {
     if constexpr (STREAM) {
 #pragma unroll
       for (int m = 0; m < E; ++m) n[c][m] = keep_or_zero(n[c][m], disk);
@@ -727,25 +836,33 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     if constexpr (STREAM) __builtin_amdgcn_sched_barrier(0);
   }
   if (!valid) return;
-  const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+  if constexpr (NC == 1) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int kz = t + TP * m;
-    const T a3 = T(tb.kd3[kz]);
-    // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
-    const T sq = a1 * a1 + a2 * a2 + a3 * a3;
-    const T den = sq == T(0) ? T(1) : sq;
-    const cpx<T> dot = cadd(cadd(cscale(n[0][m], a1), cscale(n[1][m], a2)), cscale(n[2][m], a3));
-    const cpx<T> lon = mk<T>(dot.x / den, dot.y / den);
-    const cpx<T> p1 = csub(n[0][m], cscale(lon, a1));
-    const cpx<T> p2 = csub(n[1][m], cscale(lon, a2));
-    const cpx<T> p3 = csub(n[2][m], cscale(lon, a3));
-    const long long o = (long long)kz * plane + p5;
-    // component-wise selects (a select of whole complex values is lowered
-    // through a private array in FP64)
-    nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
-    nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
-    nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+    for (int m = 0; m < E; ++m) {
+      const long long o = (long long)(t + TP * m) * plane + p5;
+      nh[o] = mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0));
+    }
+  } else {
+    const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int kz = t + TP * m;
+      const T a3 = T(tb.kd3[kz]);
+      // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
+      const T sq = a1 * a1 + a2 * a2 + a3 * a3;
+      const T den = sq == T(0) ? T(1) : sq;
+      const cpx<T> dot = cadd(cadd(cscale(n[0][m], a1), cscale(n[1][m], a2)), cscale(n[2][m], a3));
+      const cpx<T> lon = mk<T>(dot.x / den, dot.y / den);
+      const cpx<T> p1 = csub(n[0][m], cscale(lon, a1));
+      const cpx<T> p2 = csub(n[1][m], cscale(lon, a2));
+      const cpx<T> p3 = csub(n[2][m], cscale(lon, a3));
+      const long long o = (long long)kz * plane + p5;
+      // component-wise selects (a select of whole complex values is lowered
+      // through a private array in FP64)
+      nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
+      nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
+      nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+    }
   }
 }
 
@@ -913,7 +1030,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
+    hipLaunchKernelGGL((k_uxw_y<T, N, 1, 6>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
                        tb, twy, nt);
   });
   if ((st = check_launch("uxw K2"))) return st;
@@ -926,10 +1043,10 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     // stage the next pair through LDS when a pencil sits in one wave and the
     // kept kx range fits the staging area
     if (C::XSYNC && g.nhk <= XStage<T, N>::CAP)
-      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
+      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
                          s, W2, W1, g, twx, T(1.0 / P), nt);
     else
-      hipLaunchKernelGGL((k_uxw_x<T, N, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2,
+      hipLaunchKernelGGL((k_uxw_x<T, N, false, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2,
                          W1, g, twx, T(1.0 / P), nt);
   });
   if ((st = check_launch("uxw K3"))) return st;
@@ -938,7 +1055,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, -1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
+    hipLaunchKernelGGL((k_uxw_y<T, N, -1, 3>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
                        tb, twy, nt);
   });
   if ((st = check_launch("uxw K4"))) return st;
@@ -950,7 +1067,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     const TileSet* ts = tile_set(p, g, C::B, kc, s);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2,
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 3>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2,
                          out, g, tb, twz, (const int*)nullptr, nt);
       return;
     }
@@ -961,7 +1078,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
                          plane);
     }
     if (ts->nd > 0)
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 3>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
                          W2, out, g, tb, twz, (const int*)ts->dev, (long long)ts->nd);
   });
   if ((st = check_launch("uxw K5"))) return st;
@@ -991,8 +1108,120 @@ gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsi
   return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, s);
 }
 
+// Fused scalar product p^ = M F(F^-1(M a^) F^-1(M b^)), the five passes of
+// u x omega with two fields in and one out (15 M s_c at the fused ideal):
+//   P1 z-inverse of a^, b^ on the disk pencils -> W1 [2][Nz][nyc][nhp]
+//   P2 y-inverse of 2 fields                    -> W2 [2][Nz][Ny][nhp]
+//   P3 x: (a, b) of a row in one complex C2R, p = a b, p of two rows in one
+//      R2C                                      -> W1 [1][Nz][Ny][nhp]
+//   P4 y-forward of 1 field, disk ky            -> W2 [1][Nz][nyc][nhp]
+//   P5 zero tiles + z-forward of 1 field, mask  -> p^ (full layout)
+static size_t dp_work_bytes(const Geo& g, size_t cb) {
+  const size_t b1 = (size_t)(2 * g.s1 > g.s2 ? 2 * g.s1 : g.s2) * cb;
+  return b1 + (size_t)2 * g.s2 * cb;
+}
+
+template <typename T>
+static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<T>* out,
+                           const Geo& g, double kc, hipStream_t s) {
+  const size_t cb = sizeof(cpx<T>);
+  gbg_status st = ensure_work(p, dp_work_bytes(g, cb));
+  if (st) return st;
+  cpx<T>* W1 = (cpx<T>*)p->work;
+  cpx<T>* W2 = (cpx<T>*)((char*)p->work + (size_t)(2 * g.s1 > g.s2 ? 2 * g.s1 : g.s2) * cb);
+  const Tables tb = tables_of(p);
+  const cpx<T>* twx = (const cpx<T>*)p->tw[0];
+  const cpx<T>* twy = (const cpx<T>*)p->tw[1];
+  const cpx<T>* twz = (const cpx<T>*)p->tw[2];
+  const double P = (double)g.nx * g.ny * g.nz;
+  prof_begin(p, s);
+  with_len(g.nz, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_dp_z_inverse<T, N>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, ah, bh,
+                       W1, g, tb, twz, nt);
+  });
+  if ((st = check_launch("product P1"))) return st;
+  prof_mark(p, s);
+  with_len(g.ny, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_uxw_y<T, N, 1, 2>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2,
+                       g, tb, twy, nt);
+  });
+  if ((st = check_launch("product P2"))) return st;
+  prof_mark(p, s);
+  with_len(g.nx, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
+    const long long nt = (pairs + C::U - 1) / C::U;
+    if (C::XSYNC && g.nhk <= XStage<T, N>::CAP)
+      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0, true>), dim3(fast::xcd_grid(nt)),
+                         dim3(C::XNT), 0, s, W2, W1, g, twx, T(1.0 / P), nt);
+    else
+      hipLaunchKernelGGL((k_uxw_x<T, N, false, true>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
+                         s, W2, W1, g, twx, T(1.0 / P), nt);
+  });
+  if ((st = check_launch("product P3"))) return st;
+  prof_mark(p, s);
+  with_len(g.ny, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
+    hipLaunchKernelGGL((k_uxw_y<T, N, -1, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2,
+                       g, tb, twy, nt);
+  });
+  if ((st = check_launch("product P4"))) return st;
+  prof_mark(p, s);
+  with_len(g.nz, [&](auto L) {
+    constexpr int N = decltype(L)::value;
+    using C = Cfg<T, N>;
+    const long long plane = (long long)g.ny * g.nh;
+    const TileSet* ts = tile_set(p, g, C::B, kc, s);
+    if (!ts) {
+      const long long nt = (plane + C::B - 1) / C::B;
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s,
+                         W2, out, g, tb, twz, (const int*)nullptr, nt);
+      return;
+    }
+    if (ts->nd < ts->nt) {
+      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun), (unsigned)g.nz);
+      hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, C::B,
+                         plane);
+    }
+    if (ts->nd > 0)
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 1>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0,
+                         s, W2, out, g, tb, twz, (const int*)ts->dev, (long long)ts->nd);
+  });
+  if ((st = check_launch("product P5"))) return st;
+  prof_mark(p, s);
+  return GBG_OK;
+}
+
+static bool fusable_grid(const gbg_plan* p) {
+  return !p->generic_only && fast_len(p->n[0]) && fast_len(p->n[1]) && fast_len(p->n[2]);
+}
+
+size_t dealiased_product_work_bytes(gbg_plan* p, double kc) {
+  const size_t cs = (size_t)p->elem * 2, rs = (size_t)p->elem;
+  if (!fusable_grid(p)) return 2 * p->M * cs + 2 * p->P * rs;
+  return dp_work_bytes(make_geo(p, kc), cs);
+}
+
 gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
                                   double kc, hipStream_t s) {
+  if (fusable_grid(p)) {
+    const Geo g = make_geo(p, kc);
+    if (p->elem == 8)
+      return dp_fused<double>(p, (const cpx<double>*)a_hat, (const cpx<double>*)b_hat,
+                              (cpx<double>*)p_hat, g, kc, s);
+    return dp_fused<float>(p, (const cpx<float>*)a_hat, (const cpx<float>*)b_hat,
+                           (cpx<float>*)p_hat, g, kc, s);
+  }
+  // composed: the standalone operators and transforms (non-power-of-two axes)
   const size_t cs = (size_t)p->elem * 2, rs = (size_t)p->elem;
   const size_t spec1 = p->M * cs, phys1 = p->P * rs;
   gbg_status st = ensure_work(p, 2 * spec1 + 2 * phys1);
