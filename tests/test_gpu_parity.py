@@ -77,7 +77,8 @@ def test_transforms(G, n, dtype):
 def test_transforms_generic_vs_fast(G, n, dtype, monkeypatch):
     """Power-of-two lengths take the register-resident passes; the mixed-radix
     LDS passes (GBG_GENERIC_FFT=1 at plan creation) must agree with them and
-    with the oracle."""
+    with the oracle.  Vector and tensor fields: the passes that transform
+    three components per workgroup, in one and in three groups of three."""
     say(f"[generic-vs-fast] n={n} dtype={np.dtype(dtype).name}")
     g = ogrid(n)
     grid = G.Grid3D(n)
@@ -85,14 +86,18 @@ def test_transforms_generic_vs_fast(G, n, dtype, monkeypatch):
     monkeypatch.setenv("GBG_GENERIC_FFT", "1")
     generic = G.FieldTransformPlans(grid, dtype)
     monkeypatch.delenv("GBG_GENERIC_FFT")
-    f = rand_phys(g, 1, 21)
-    ref = O.to_spectral(f)
-    phys = G.Field(grid, "physical", 1, dtype).upload(f)
-    for plans in (fast, generic):
-        spec = G.to_spectral(phys, plans)
-        assert O.rel_l2(spec.download(), ref) < TOL[dtype]
-        back = G.to_physical(spec, plans)
-        assert O.rel_l2(back.download(), f) < 10 * TOL[dtype]
+    for order in (1, 2):
+        f = rand_phys(g, order, 21 + order)
+        ref = O.to_spectral(f)
+        phys = G.Field(grid, "physical", order, dtype).upload(f)
+        for plans in (fast, generic):
+            spec = G.to_spectral(phys, plans)
+            assert O.rel_l2(spec.download(), ref) < TOL[dtype]
+            back = G.to_physical(spec, plans)
+            assert O.rel_l2(back.download(), f) < 10 * TOL[dtype]
+            out = G.Field(grid, "physical", order, dtype)
+            G.to_physical_(out, spec, plans)   # 3-arg: consumes spec
+            assert O.rel_l2(out.download(), f) < 10 * TOL[dtype]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
