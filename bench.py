@@ -83,9 +83,11 @@ def stage_bytes(n, elem):
             "K5_z_forward_project": 6 * comp}
 
 
-KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}>", "K2_y_inverse": "k_uxw_y<{t}, {n}, 1>",
-                   "K3_x_product": "k_uxw_x<{t}, {n}, true>", "K4_y_forward": "k_uxw_y<{t}, {n}, -1>",
-                   "K5_z_forward_project": ("k_uxw_zero_tiles<{t}>", "k_uxw_z_forward<{t}, {n}>")}
+KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}>",
+                   "K2_y_inverse": "k_uxw_y<{t}, {n}, 1, 6>",
+                   "K3_x_product": "k_uxw_x<{t}, {n}, true, false>",
+                   "K4_y_forward": "k_uxw_y<{t}, {n}, -1, 3>",
+                   "K5_z_forward_project": ("k_uxw_zero_tiles<{t}>", "k_uxw_z_forward<{t}, {n}, 3>")}
 
 
 def measured_traffic(stage, n, dtype):
