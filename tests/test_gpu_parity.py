@@ -265,26 +265,28 @@ def test_config1_pipeline(G, dtype):
     assert O.rel_l2(pp.download(), O.to_physical(ref_ph, g)) < tol
 
 
-DP_CASES = [((64, 64, 64), None), ((32, 64, 128), None), ((128, 32, 64), None),
-            ((64, 64, 64), 1e9), ((64, 64, 64), 3.5), ((256, 256, 256), None),
-            ((48, 40, 36), None)]
+TWO_PI3 = (O.TWO_PI,) * 3
+DP_CASES = [((64, 64, 64), None, TWO_PI3), ((32, 64, 128), None, TWO_PI3),
+            ((128, 32, 64), None, TWO_PI3), ((64, 64, 64), 1e9, TWO_PI3),
+            ((64, 64, 64), 3.5, TWO_PI3), ((64, 32, 128), 20.0, (1.0, 2.5, 7.0)),
+            ((256, 256, 256), None, TWO_PI3), ((48, 40, 36), None, TWO_PI3)]
 
 
-@pytest.mark.parametrize("n,kc", DP_CASES)
+@pytest.mark.parametrize("n,kc,L", DP_CASES)
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_dealiased_product_parity(G, n, kc, dtype, monkeypatch):
+def test_dealiased_product_parity(G, n, kc, L, dtype, monkeypatch):
     """p^ = M F(F^-1(M a^) F^-1(M b^)) against the oracle: the fused five-pass
-    form on power-of-two grids (cubic and not; the 2/3 cutoff, no truncation,
-    a cutoff of 3.5), the composed form on the others and on a
+    form on power-of-two grids (cubic and not, an anisotropic box; the 2/3
+    cutoff, no truncation, a cutoff of 3.5), the composed form on the others and on a
     plan restricted to the mixed-radix passes; both agree with each other; the
     removed modes are exactly zero; a^ = b^ (the square) is allowed."""
-    say(f"[dealiased product] n={n} kc={kc} dtype={np.dtype(dtype).name}")
-    g = ogrid(n)
+    say(f"[dealiased product] n={n} kc={kc} L={L} dtype={np.dtype(dtype).name}")
+    g = ogrid(n, L)
     m = O.Multipliers.build(g)
     kc = O.dealias_cutoff(n[0]) if kc is None else kc
     a_hat = O.synthetic_field(g, 1001, m=m)
     b_hat = O.synthetic_field(g, 1002, m=m)
-    grid = G.Grid3D(n)
+    grid = G.Grid3D(n, L)
     plans = G.FieldTransformPlans(grid, dtype)
     monkeypatch.setenv("GBG_GENERIC_FFT", "1")
     composed = G.FieldTransformPlans(grid, dtype)
