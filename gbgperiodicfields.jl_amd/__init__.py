@@ -721,6 +721,20 @@ class Server:
             self.inflight.pop(ticket, None)
         return ms.value
 
+    def poll(self, ticket: int):
+        """The job's latency (ms) once it has finished, else None (non-blocking)."""
+        done, ms = C.c_int(0), C.c_double(0.0)
+        try:
+            check(load().gbg_server_poll(self.handle, C.c_ulonglong(ticket), C.byref(done),
+                                         C.byref(ms)))
+        except Exception:
+            self.inflight.pop(ticket, None)
+            raise
+        if not done.value:
+            return None
+        self.inflight.pop(ticket, None)
+        return ms.value
+
     def stats(self):
         p50, p99, tput = C.c_double(), C.c_double(), C.c_double()
         n = C.c_ulonglong()

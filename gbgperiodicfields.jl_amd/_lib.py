@@ -83,6 +83,7 @@ SIGNATURES = {
     "gbg_server_create": (st, [sz, i32, C.POINTER(vp)]),
     "gbg_server_submit": (st, [vp, vp, C.POINTER(C.c_ulonglong)]),
     "gbg_server_wait": (st, [vp, C.c_ulonglong, dp]),
+    "gbg_server_poll": (st, [vp, C.c_ulonglong, C.POINTER(C.c_int), dp]),
     "gbg_server_stats": (st, [vp, dp, dp, dp, C.POINTER(C.c_ulonglong)]),
     "gbg_server_resident": (st, [vp, C.POINTER(sz), C.POINTER(C.c_int)]),
     "gbg_server_destroy": (st, [vp]),

@@ -334,6 +334,27 @@ gbg_status gbg_server_wait(gbg_server* s, unsigned long long ticket, double* lat
   return GBG_OK;
 }
 
+gbg_status gbg_server_poll(gbg_server* s, unsigned long long ticket, int* done,
+                           double* latency_ms) {
+  if (!s) return fail(GBG_E_ARG, "null server");
+  if (!done) return fail(GBG_E_ARG, "null done flag");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(ticket);
+  if (it == s->jobs.end()) return fail(GBG_E_ARG, "unknown ticket");
+  ServeJob* j = it->second;
+  if (!(j->done && j->dispatched)) {
+    *done = 0;
+    return GBG_OK;
+  }
+  *done = 1;
+  const gbg_status st = j->status;
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count();
+  s->jobs.erase(it);
+  delete j;
+  if (st) return fail(st, "job failed");
+  return GBG_OK;
+}
+
 gbg_status gbg_server_stats(gbg_server* s, double* p50_ms, double* p99_ms, double* jobs_per_s,
                             unsigned long long* completed) {
   if (!s) return fail(GBG_E_ARG, "null server");

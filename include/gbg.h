@@ -264,6 +264,11 @@ gbg_status gbg_server_create(size_t hbm_budget, int slots, gbg_server** out);
 gbg_status gbg_server_submit(gbg_server* server, const gbg_job* job, unsigned long long* ticket);
 /* Blocks until the job is done; *latency_ms = completion - submission. */
 gbg_status gbg_server_wait(gbg_server* server, unsigned long long ticket, double* latency_ms);
+/* Returns at once: *done = 0 while the job runs (the ticket stays valid);
+ * *done = 1 once it has finished, with the job's status and *latency_ms as
+ * gbg_server_wait gives them (the ticket is then retired). */
+gbg_status gbg_server_poll(gbg_server* server, unsigned long long ticket, int* done,
+                           double* latency_ms);
 /* Latency percentiles over completed jobs and their throughput since the
  * first submission. */
 gbg_status gbg_server_stats(gbg_server* server, double* p50_ms, double* p99_ms, double* jobs_per_s,

@@ -3,6 +3,7 @@ both match the oracle, admission refuses jobs beyond the budget, and a tight
 budget evicts idle plans without changing results."""
 
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -90,6 +91,23 @@ def test_serve_alone_equals_mix(G):
         assert np.array_equal(out.download(), alone[k]), f"request {k} differs in the mix"
     st = srv.stats()
     assert st["completed"] == len(subs) and st["p99_ms"] >= st["p50_ms"] > 0
+    # non-blocking completion: poll each ticket until it reports done
+    subs = []
+    for k in range(len(reqs)):
+        kind, mk, i0, i1, kc, _ = reqs[k]
+        out = mk()
+        subs.append((k, out, srv.submit(kind, out, i0, i1, kc)))
+    for k, out, t in subs:
+        lat = None
+        for _ in range(200000):
+            lat = srv.poll(t)
+            if lat is not None:
+                break
+            time.sleep(0.0005)
+        assert lat is not None and lat > 0, f"request {k} never completed"
+        assert np.array_equal(out.download(), alone[k])
+    with pytest.raises(G.ArgumentError):
+        srv.poll(subs[0][2])          # the ticket is retired once reported done
     srv.close()
 
 
