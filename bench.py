@@ -509,8 +509,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the spectral path here runs on a single MI355X and has "
-              "no multi-GPU form (see DESIGN.md for how it would shard). Run with --gpus 1.",
-              file=sys.stderr)
+              "no multi-GPU form (a field would shard as z slabs with one all-to-all per "
+              "transform direction). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     G = load_package()
