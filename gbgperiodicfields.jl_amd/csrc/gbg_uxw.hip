@@ -867,21 +867,32 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 }
 
 // The output tiles that hold no kept (kx, ky) pencil: zeros in all kz planes
-// and components.  Workgroup (x, y) covers kZeroRun consecutive pencil indices
-// of one (component, kz) plane y, so each workgroup streams one contiguous
-// stretch of the output.
+// and components.  Workgroup (x, y) covers kZeroRun consecutive pencil
+// indices of kZeroPlanes consecutive (component, kz) planes: the tile flags of
+// its stretch are read once, then each plane's stretch is streamed out with no
+// load between the stores.
 constexpr int kZeroRun = 2048;
+constexpr int kZeroPlanes = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
                                                         const int* __restrict__ flag, int b,
-                                                        long long plane) {
-  cpx<T>* out = nh + (long long)blockIdx.y * plane;
+                                                        long long plane, int nplanes) {
+  constexpr int R = kZeroRun / 256;
   const cpx<T> zero = mk<T>(T(0), T(0));
   const long long base = (long long)blockIdx.x * kZeroRun + threadIdx.x;
+  bool z[R];
 #pragma unroll
-  for (int i = 0; i < kZeroRun / 256; ++i) {
+  for (int i = 0; i < R; ++i) {
     const long long p5 = base + i * 256;
-    if (p5 < plane && !flag[p5 / b]) out[p5] = zero;
+    z[i] = p5 < plane && !flag[p5 / b];
+  }
+  const int p0 = blockIdx.y * kZeroPlanes;
+  const int p1 = p0 + kZeroPlanes < nplanes ? p0 + kZeroPlanes : nplanes;
+  for (int pl = p0; pl < p1; ++pl) {
+    cpx<T>* out = nh + (long long)pl * plane;
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (z[i]) out[base + i * 256] = zero;
   }
 }
 
@@ -1073,9 +1084,11 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     }
     if (ts->nd < ts->nt) {
       // planes (component c, kz) are consecutive: c M + kz plane = (c nz + kz) plane
-      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun), (unsigned)(3 * g.nz));
+      const int np = 3 * g.nz;
+      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun),
+                    (unsigned)((np + kZeroPlanes - 1) / kZeroPlanes));
       hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, C::B,
-                         plane);
+                         plane, np);
     }
     if (ts->nd > 0)
       hipLaunchKernelGGL((k_uxw_z_forward<T, N, 3>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
@@ -1188,9 +1201,10 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
       return;
     }
     if (ts->nd < ts->nt) {
-      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun), (unsigned)g.nz);
+      const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun),
+                    (unsigned)((g.nz + kZeroPlanes - 1) / kZeroPlanes));
       hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, C::B,
-                         plane);
+                         plane, g.nz);
     }
     if (ts->nd > 0)
       hipLaunchKernelGGL((k_uxw_z_forward<T, N, 1>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0,
