@@ -5,7 +5,10 @@
 //
 // Strided passes load B consecutive pencils of the fast index into LDS, so
 // every global row segment is B contiguous complex elements; the x passes
-// pair two real rows into one complex transform (z = a + i b).
+// pair two real rows into one complex transform (z = a + i b).  Tiles map to
+// workgroups XCD-aware (fast::xcd_tile): with the odd row pitch N/2 + 1 a
+// tile's segments straddle cache lines, and the neighbouring tile that shares
+// them runs on the same XCD, behind the same L2.
 #include "fft_fast.hpp"
 
 namespace gbg {
@@ -26,7 +29,7 @@ __global__ __launch_bounds__(kNT) void k_c2c_strided(const cpx<T>* in, cpx<T>* o
   const int pl = threadIdx.x & (d.B - 1);
   const int e0 = threadIdx.x >> d.bshift;
   const int estep = kNT >> d.bshift;
-  const long long p = (long long)blockIdx.x * d.B + pl;
+  const long long p = fast::xcd_tile() * d.B + pl;
   const bool valid = p < d.npencil;
   const long long base = coff + (valid ? (p / d.inner) * d.ostride + (p % d.inner) : 0);
   const int n = ap.n;
@@ -49,7 +52,7 @@ __global__ __launch_bounds__(kNT) void k_r2c_x(const T* __restrict__ in, cpx<T>*
                                                long long nrows, int nx, int nh, int B, int ld,
                                                AxisPlan ap, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> s[2 * Lds<T>::kElems];
-  const long long pair0 = (long long)blockIdx.x * B;
+  const long long pair0 = fast::xcd_tile() * B;
   for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
     const int q = idx / nx, x = idx - q * nx;
     const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(kNT) void k_c2r_x(const cpx<T>* __restrict__ in, T*
                                                long long nrows, int nx, int nh, int B, int ld,
                                                AxisPlan ap, const cpx<T>* __restrict__ tw, T scale) {
   __shared__ cpx<T> s[2 * Lds<T>::kElems];
-  const long long pair0 = (long long)blockIdx.x * B;
+  const long long pair0 = fast::xcd_tile() * B;
   const int nyq = (nx % 2 == 0) ? nx / 2 : -1;
   for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
     const int q = idx / nx, k = idx - q * nx;
@@ -129,7 +132,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_fast_c2c(const cpx<T>* in, 
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const long long p = (long long)blockIdx.x * B + pl;
+  const long long p = fast::xcd_tile() * B + pl;
   const bool valid = p < d.npencil;
   const long long base =
       (long long)blockIdx.y * d.cstride + (valid ? (p / d.inner) * d.ostride + (p % d.inner) : 0);
@@ -159,7 +162,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __rest
   constexpr int E = C::XE, TP = C::XTP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
-  const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
+  const long long r0 = 2 * (fast::xcd_tile() * C::U + un), r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
   cpx<T> v[E];
 #pragma unroll
@@ -184,7 +187,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_c2r_x(const cpx<T>* _
   constexpr int E = C::XE, TP = C::XTP, LD = C::XLD;
   __shared__ cpx<T> lds[2 * C::U * LD];
   const int un = threadIdx.x / TP, t = threadIdx.x % TP;
-  const long long r0 = 2 * ((long long)blockIdx.x * C::U + un), r1 = r0 + 1;
+  const long long r0 = 2 * (fast::xcd_tile() * C::U + un), r1 = r0 + 1;
   const bool ok0 = r0 < nrows, ok1 = r1 < nrows;
   cpx<T> v[E];
   fast::c2r_pair_load<T, N, E>(v, in + (ok0 ? r0 * g.nh : 0), in + (ok1 ? r1 * g.nh : 0), ok0, ok1,
@@ -206,7 +209,7 @@ template <typename T, int N>
 static void launch_fast_c2c(int dir, const cpx<T>* in, cpx<T>* out, const Strided& d, int ncomp,
                             const cpx<T>* tw, T scale, hipStream_t s) {
   using C = Cfg<T, N>;
-  dim3 grid((unsigned)((d.npencil + C::B - 1) / C::B), (unsigned)ncomp);
+  dim3 grid(fast::xcd_grid((d.npencil + C::B - 1) / C::B), (unsigned)ncomp);
   if (dir < 0)
     hipLaunchKernelGGL((k_fast_c2c<T, N, -1>), grid, dim3(C::NT), 0, s, in, out, d, tw, scale);
   else
@@ -232,7 +235,7 @@ static void launch_fast_x(int dir, const void* in, void* out, long long nrows, R
                           const cpx<T>* tw, T scale, hipStream_t s) {
   using C = Cfg<T, N>;
   const long long pairs = (nrows + 1) / 2;
-  dim3 grid((unsigned)((pairs + C::U - 1) / C::U));
+  dim3 grid(fast::xcd_grid((pairs + C::U - 1) / C::U));
   if (dir < 0)
     hipLaunchKernelGGL((k_fast_r2c_x<T, N>), grid, dim3(C::XNT), 0, s, (const T*)in, (cpx<T>*)out,
                        nrows, g, tw);
@@ -290,7 +293,7 @@ static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* i
   if (!p->generic_only && p->use_fast[axis] &&
       try_fast_c2c<T>(ap.n, dir, in, out, d, ncomp, tw, scale, s))
     return check_launch("c2c pass");
-  dim3 grid((unsigned)((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
+  dim3 grid(fast::xcd_grid((d.npencil + d.B - 1) / d.B), (unsigned)ncomp);
   if (dir < 0)
     hipLaunchKernelGGL((k_c2c_strided<T, -1>), grid, dim3(kNT), 0, s, in, out, d, ap, tw, scale);
   else
@@ -308,7 +311,7 @@ static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, 
       !try_fast_x<T>(nx, -1, phys, spec, nrows, rg, twx, T(1), s)) {
     const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
-    dim3 grid((unsigned)((pairs + B - 1) / B));
+    dim3 grid(fast::xcd_grid((pairs + B - 1) / B));
     hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
                        p->ax[0], twx);
   }
@@ -333,7 +336,7 @@ static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, 
       !try_fast_x<T>(nx, 1, work, phys, nrows, rg, twx, scale, s)) {
     const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
-    dim3 grid((unsigned)((pairs + B - 1) / B));
+    dim3 grid(fast::xcd_grid((pairs + B - 1) / B));
     hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,
                        p->ax[0], twx, scale);
   }
