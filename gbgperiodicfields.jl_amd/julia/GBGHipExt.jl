@@ -94,6 +94,12 @@ end
 gradient!(dst, src, plan::GBGPlan) =
     (check(ccall((:gbg_gradient, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), field_tensor_order(src), C_NULL)); dst)
+divergence!(dst, src, plan::GBGPlan) =
+    (check(ccall((:gbg_divergence, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Cdouble, Cdouble,
+                  Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(src), field_tensor_order(src), 0, 0.0, 0.0, 1.0, 0.0,
+                 C_NULL)); dst)
 curl!(dst, src, plan::GBGPlan) =
     (check(ccall((:gbg_curl, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), C_NULL)); dst)
@@ -122,6 +128,16 @@ mean_free_projection!(f, plan::GBGPlan) =
 cross!(dst, u, v, plan::GBGPlan) =
     (check(ccall((:gbg_cross, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(u), dptr(v), C_NULL)); dst)
+multiply!(dst, a, b, plan::GBGPlan) =
+    (check(ccall((:gbg_multiply, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(a), dptr(b), C_NULL)); dst)
+function inner(f::AbstractFieldPhysical, g::AbstractFieldPhysical, plan::GBGPlan)
+    out = Ref{Cdouble}(0.0)
+    check(ccall((:gbg_inner_physical, libgbg), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Cdouble}, Ptr{Cvoid}),
+                plan.handle, dptr(f), dptr(g), ncomp(f), out, C_NULL))
+    return out[]
+end
 function inner(f::AbstractFieldSpectral, g::AbstractFieldSpectral, plan::GBGPlan)
     out = Ref{Cdouble}(0.0)
     check(ccall((:gbg_inner_spectral, libgbg), Cint,
@@ -174,6 +190,20 @@ reality_projection!(dst, src, plan::GBGPlan) =
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), ncomp(src), C_NULL)); dst)
 reality_projection!(f, plan::GBGPlan) = reality_projection!(f, f, plan)
+
+# ---- the consumer's dealiased product and the planner's role -----------------------------
+# p^ = M F(F^-1(M a^) F^-1(M b^)) for scalar fields (fused on power-of-two grids)
+dealiased_product!(ph, ah, bh, plan::GBGPlan, kc) =
+    (check(ccall((:gbg_dealiased_product, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+                 plan.handle, dptr(ph), dptr(ah), dptr(bh), kc, C_NULL)); ph)
+# per-axis pass choice, the role of FFTW's planner flags; returns forward + inverse ms
+function autotune!(plan::GBGPlan)
+    ms = Ref{Cdouble}(0.0)
+    check(ccall((:gbg_plan_autotune, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Cdouble}),
+                plan.handle, C_NULL, ms))
+    return ms[]
+end
 
 # ---- the fused right-hand side of examples/HIST_CPU.jl:229-247 --------------------------
 function rhs_uxw!(Nh, uh, plan::GBGPlan, kc)
