@@ -169,7 +169,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     if constexpr (TWL)
       return (const cpx<T>*)twl;
     else
-      return fresh<true>(tw);
+      return tw;
   };
   const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
   const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
