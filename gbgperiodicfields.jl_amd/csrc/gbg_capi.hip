@@ -196,6 +196,9 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
   p->M = M;
   p->P = (size_t)n[0] * n[1] * n[2];
   gbg_status st = check_hip(hipGetDevice(&p->device), "hipGetDevice");
+  if (!st)
+    st = check_hip(hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, p->device),
+                   "hipDeviceGetAttribute");
   if (st) {
     destroy_plan(p);
     return st;

@@ -102,6 +102,7 @@ struct gbg_plan {
   size_t M;          // stored modes per component
   size_t P;          // grid points per component
   int device;
+  int ncu;           // compute units of that device
   int generic_only;  // GBG_GENERIC_FFT=1 at creation: mixed-radix LDS passes only
   gbg::AxisPlan ax[3];
   // per-axis transform choice (set by gbg_plan_autotune): use_fast = take the

@@ -126,29 +126,6 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   __shared__ double red[2][C::NT / 64];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const long long tile = fast::xcd_tile();
-  const long long p1 = tile * B + pl;
-  int kx = 0, ky = 0;
-  double qxy = 0.0;
-  bool disk = false;
-  if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
-    const int kyc = (int)(p1 / g.nhp);
-    kx = (int)(p1 - (long long)kyc * g.nhp);
-    if (kx < g.nhk) {
-      ky = ky_of(kyc, g);
-      qxy = tb.ks1[kx] + tb.ks2[ky];
-      disk = qxy <= g.hi2;
-    } else {
-      kx = 0;
-    }
-  }
-  if (!__syncthreads_or(disk)) {
-    if (threadIdx.x == 0) {
-      part[2 * blockIdx.x] = 0.0;
-      part[2 * blockIdx.x + 1] = 0.0;
-    }
-    return;
-  }
   // The transforms' twiddles and the kz derivative factors are read from LDS
   // copies: a global load consumed inside the loop below would wait for every
   // load issued before it (the counter of outstanding vector-memory loads
@@ -171,116 +148,150 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     else
       return tw;
   };
-  const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
-  const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
-  const long long plane = (long long)g.ny * g.nh;
-  const long long off = (long long)ky * g.nh + kx;
-  bool keep[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + tb.ks3[t + TP * m] <= g.hi2);  // no branch
   double eacc = 0.0, zacc = 0.0;
-  cpx<T>* pen = lds + pl * LD;
-  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
-  const long long zs = (long long)g.nyc * g.nhp;
-  // Six output fields from four component loads, each issued at least one
-  // transform before it is needed, with two raw components X, Y held and no
-  // register copies between them:
-  //   u1 | w2 = i(a3 X - a1 Y), X <- u2 | u3 | w1 = i(a2 Y - a3 X), Y <- u1 | u2 | w3 = i(a1 X - a2 Y)
-  // starting from X = u1, Y = u3.  Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
-  // Entries outside the ball load the pencil's kz = 0 entry instead (a valid
-  // address; on the disk a line the kept lanes fetch anyway) and are zeroed
-  // where they are used.  No branch surrounds a load or a store, so the
-  // compiler's wait for one load never has to cover another issued after it
-  // and each load stays in flight across the transform that follows it.
-  // Every lane of a tile that holds a disk pencil stores (zeros off the disk):
-  // with B dividing the row pitch nhp, all of them address the buffer.
-  static_assert(16 % B == 0 || B > 16, "tile width");
-  constexpr bool ALL_STORE = 16 % B == 0;
-  const bool st_ok = ALL_STORE || p1 < (long long)g.nyc * g.nhp;
-  cpx<T> X[E], Y[E];
-  auto loadc = [&](cpx<T> (&r)[E], int c) {
-    const cpx<T>* u = uh + c * g.M + off;
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      long long zi = keep[m] ? (long long)(t + TP * m) * plane : 0;
-      asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
-      r[m] = u[zi];
+  // one tile: B consecutive kx of one compact (kyc) row
+  auto run_tile = [&](long long tile) {
+    const long long p1 = tile * B + pl;
+    int kx = 0, ky = 0;
+    double qxy = 0.0;
+    bool disk = false;
+    if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
+      const int kyc = (int)(p1 / g.nhp);
+      kx = (int)(p1 - (long long)kyc * g.nhp);
+      if (kx < g.nhk) {
+        ky = ky_of(kyc, g);
+        qxy = tb.ks1[kx] + tb.ks2[ky];
+        disk = qxy <= g.hi2;
+      } else {
+        kx = 0;
+      }
     }
-  };
-  auto emit = [&](cpx<T> (&v)[E], int field) {
-    fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
-    if (st_ok) {
-      cpx<T>* o = w1 + field * g.s1 + p1;
+    if (!__syncthreads_or(disk)) return;
+    double ea = 0.0, za = 0.0;   // this tile's sums (local: no pointer to the outer ones)
+    const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+    const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
+    const long long plane = (long long)g.ny * g.nh;
+    const long long off = (long long)ky * g.nh + kx;
+    bool keep[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
-    }
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-  };
-  auto velocity = [&](const cpx<T> (&A)[E], int field) {
-    cpx<T> v[E];
+    for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + tb.ks3[t + TP * m] <= g.hi2);  // no branch
+    cpx<T>* pen = lds + pl * LD;
+    cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
+    const long long zs = (long long)g.nyc * g.nhp;
+    // Six output fields from four component loads, each issued at least one
+    // transform before it is needed, with two raw components X, Y held and no
+    // register copies between them:
+    //   u1 | w2 = i(a3 X - a1 Y), X <- u2 | u3 | w1 = i(a2 Y - a3 X), Y <- u1 | u2 | w3 = i(a1 X - a2 Y)
+    // starting from X = u1, Y = u3.  Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
+    // Entries outside the ball load the pencil's kz = 0 entry instead (a valid
+    // address; on the disk a line the kept lanes fetch anyway) and are zeroed
+    // where they are used.  No branch surrounds a load or a store, so the
+    // compiler's wait for one load never has to cover another issued after it
+    // and each load stays in flight across the transform that follows it.
+    // Every lane of a tile that holds a disk pencil stores (zeros off the disk):
+    // with B dividing the row pitch nhp, all of them address the buffer.
+    static_assert(16 % B == 0 || B > 16, "tile width");
+    constexpr bool ALL_STORE = 16 % B == 0;
+    const bool st_ok = ALL_STORE || p1 < (long long)g.nyc * g.nhp;
+    cpx<T> X[E], Y[E];
+    auto loadc = [&](cpx<T> (&r)[E], int c) {
+      const cpx<T>* u = uh + c * g.M + off;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      v[m] = keep_or_zero(A[m], keep[m]);
-      eacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
-    }
-    emit(v, field);
-  };
-  // w = i(ca A - cb B), ia / ib = 1, 2, 3 naming which of a1, a2, a3 ca and cb are;
-  // then the next component (nc >= 0) loads into A
-  auto vorticity = [&](cpx<T> (&A)[E], const cpx<T> (&Bv)[E], int ia, int ib, int field, int nc) {
-    cpx<T> v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const T a3 = a3l[t + TP * m];
-      const T ca = ia == 1 ? a1 : (ia == 2 ? a2 : a3);
-      const T cb = ib == 1 ? a1 : (ib == 2 ? a2 : a3);
-      v[m] = keep_or_zero(cmuli(csub(cscale(A[m], ca), cscale(Bv[m], cb))), keep[m]);
-      zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
-    }
-    if (nc >= 0) loadc(A, nc);
-    emit(v, field);
-  };
-  // The six transforms written out one after another hold more registers
-  // than a loop over them; where that spills (short pencils, where more
-  // pencils share a workgroup), a loop over the fields loads the components
-  // of field f + 1 (one or two, nine in all) while field f transforms.
-  constexpr bool UNROLLED = sizeof(T) == 8 ? N >= 256 : N >= 512;
-  if constexpr (UNROLLED) {
-    loadc(X, 0);
-    loadc(Y, 2);
-    velocity(X, 0);
-    vorticity(X, Y, 3, 1, 4, 1);
-    velocity(Y, 2);
-    vorticity(Y, X, 2, 3, 3, 0);
-    velocity(X, 1);
-    vorticity(X, Y, 1, 2, 5, -1);
-  } else {
-    // field f: f = 0, 1, 2 -> u_f;  f = 3, 4, 5 -> w = i (ca u_ia - cb u_ib)
-    auto load = [&](int f) {
-      loadc(X, f < 3 ? f : (f == 3 ? 2 : (f == 4 ? 0 : 1)));
-      if (f >= 3) loadc(Y, f == 3 ? 1 : (f == 4 ? 2 : 0));
+      for (int m = 0; m < E; ++m) {
+        long long zi = keep[m] ? (long long)(t + TP * m) * plane : 0;
+        asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+        r[m] = u[zi];
+      }
     };
-    load(0);
-#pragma unroll 1
-    for (int f = 0; f < 6; ++f) {
+    auto emit = [&](cpx<T> (&v)[E], int field) {
+      fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
+      if (st_ok) {
+        cpx<T>* o = w1 + field * g.s1 + p1;
+#pragma unroll
+        for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
+      }
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    auto velocity = [&](const cpx<T> (&A)[E], int field) {
       cpx<T> v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        if (f < 3) {
-          v[m] = keep_or_zero(X[m], keep[m]);
-          eacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
-        } else {
-          const T a3 = a3l[t + TP * m];
-          const T ca = f == 3 ? a2 : (f == 4 ? a3 : a1);
-          const T cb = f == 3 ? a3 : (f == 4 ? a1 : a2);
-          v[m] = keep_or_zero(cmuli(csub(cscale(X[m], ca), cscale(Y[m], cb))), keep[m]);
-          zacc += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
-        }
+        v[m] = keep_or_zero(A[m], keep[m]);
+        ea += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
       }
-      if (f + 1 < 6) load(f + 1);
-      emit(v, f);
+      emit(v, field);
+    };
+    // w = i(ca A - cb B), ia / ib = 1, 2, 3 naming which of a1, a2, a3 ca and cb are;
+    // then the next component (nc >= 0) loads into A
+    auto vorticity = [&](cpx<T> (&A)[E], const cpx<T> (&Bv)[E], int ia, int ib, int field, int nc) {
+      cpx<T> v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T a3 = a3l[t + TP * m];
+        const T ca = ia == 1 ? a1 : (ia == 2 ? a2 : a3);
+        const T cb = ib == 1 ? a1 : (ib == 2 ? a2 : a3);
+        v[m] = keep_or_zero(cmuli(csub(cscale(A[m], ca), cscale(Bv[m], cb))), keep[m]);
+        za += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+      }
+      if (nc >= 0) loadc(A, nc);
+      emit(v, field);
+    };
+    // The six transforms written out one after another hold more registers
+    // than a loop over them; where that spills (short pencils, where more
+    // pencils share a workgroup), a loop over the fields loads the components
+    // of field f + 1 (one or two, nine in all) while field f transforms.
+    constexpr bool UNROLLED = sizeof(T) == 8 ? N >= 256 : N >= 512;
+    if constexpr (UNROLLED) {
+      loadc(X, 0);
+      loadc(Y, 2);
+      velocity(X, 0);
+      vorticity(X, Y, 3, 1, 4, 1);
+      velocity(Y, 2);
+      vorticity(Y, X, 2, 3, 3, 0);
+      velocity(X, 1);
+      vorticity(X, Y, 1, 2, 5, -1);
+    } else {
+      // field f: f = 0, 1, 2 -> u_f;  f = 3, 4, 5 -> w = i (ca u_ia - cb u_ib)
+      auto load = [&](int f) {
+        loadc(X, f < 3 ? f : (f == 3 ? 2 : (f == 4 ? 0 : 1)));
+        if (f >= 3) loadc(Y, f == 3 ? 1 : (f == 4 ? 2 : 0));
+      };
+      load(0);
+#pragma unroll 1
+      for (int f = 0; f < 6; ++f) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          if (f < 3) {
+            v[m] = keep_or_zero(X[m], keep[m]);
+            ea += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+          } else {
+            const T a3 = a3l[t + TP * m];
+            const T ca = f == 3 ? a2 : (f == 4 ? a3 : a1);
+            const T cb = f == 3 ? a3 : (f == 4 ? a1 : a2);
+            v[m] = keep_or_zero(cmuli(csub(cscale(X[m], ca), cscale(Y[m], cb))), keep[m]);
+            za += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+          }
+        }
+        if (f + 1 < 6) load(f + 1);
+        emit(v, f);
+      }
     }
+    eacc += ea;
+    zacc += za;
+  };
+  constexpr bool PERSIST = sizeof(T) == 8 && N >= 128 && N <= 512;
+  if constexpr (PERSIST) {
+    const long long per = (ntiles + fast::kXcd - 1) / fast::kXcd;
+    const int xcd = blockIdx.x % fast::kXcd;
+    const long long L = gridDim.x / fast::kXcd;
+    const long long tend = (xcd + 1) * per < ntiles ? (xcd + 1) * per : ntiles;
+#pragma unroll 1
+    for (long long tile = xcd * per + blockIdx.x / fast::kXcd; tile < tend; tile += L)
+      run_tile(tile);
+  } else {
+    run_tile(fast::xcd_tile());
   }
   // FP64 block reduction of the energy / enstrophy partial sums, fixed order
   for (int s = 32; s > 0; s >>= 1) {
@@ -1030,7 +1041,11 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
-    const unsigned grid = fast::xcd_grid(nt);
+    unsigned grid = fast::xcd_grid(nt);
+    // the persistent form (FP64, 128 <= Nz <= 512) runs one workgroup per CU
+    if (sizeof(T) == 8 && N >= 128 && N <= 512 && p->ncu >= fast::kXcd &&
+        grid > (unsigned)(p->ncu / fast::kXcd * fast::kXcd))
+      grid = (unsigned)(p->ncu / fast::kXcd * fast::kXcd);
     hipLaunchKernelGGL((k_uxw_z_inverse<T, N>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g, tb, twz,
                        part, nt);
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
