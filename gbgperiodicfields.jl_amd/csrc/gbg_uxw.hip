@@ -117,6 +117,13 @@ __host__ __device__ __forceinline__ int kyc_of(int ky, const Geo& g) {
 // Tiles: B consecutive kx of one compact (kyc) row, pencil index
 // p1 = kyc * nhp + kx.  Energy and enstrophy partials land in part[2 b],
 // part[2 b + 1] for the launched workgroup b (zeros for idle ones).
+// Persistent form of K1 (see k_uxw_z_inverse): FP64 at the lengths where the
+// tile loop holds in 256 VGPRs without spilling.
+template <typename T, int N>
+constexpr bool z_inverse_persist() {
+  return sizeof(T) == 8 && N >= 128 && N <= 512;
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ w1, Geo g, Tables tb,
@@ -281,7 +288,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     eacc += ea;
     zacc += za;
   };
-  constexpr bool PERSIST = sizeof(T) == 8 && N >= 128 && N <= 512;
+  constexpr bool PERSIST = z_inverse_persist<T, N>();
   if constexpr (PERSIST) {
     const long long per = (ntiles + fast::kXcd - 1) / fast::kXcd;
     const int xcd = blockIdx.x % fast::kXcd;
@@ -908,6 +915,15 @@ __global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
 }
 
 // ---- host side ----------------------------------------------------------------
+// grid of a pass over nt tiles: one workgroup per tile (XCD-padded), or for a
+// persistent pass at most one per CU (a multiple of the XCD count)
+static unsigned pass_grid(const gbg_plan* p, long long nt, bool persist) {
+  unsigned grid = fast::xcd_grid(nt);
+  const unsigned cap = (unsigned)(p->ncu / fast::kXcd * fast::kXcd);
+  if (persist && cap >= (unsigned)fast::kXcd && grid > cap) grid = cap;
+  return grid;
+}
+
 static bool fast_len(int n) {
   return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
 }
@@ -1041,11 +1057,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
-    unsigned grid = fast::xcd_grid(nt);
-    // the persistent form (FP64, 128 <= Nz <= 512) runs one workgroup per CU
-    if (sizeof(T) == 8 && N >= 128 && N <= 512 && p->ncu >= fast::kXcd &&
-        grid > (unsigned)(p->ncu / fast::kXcd * fast::kXcd))
-      grid = (unsigned)(p->ncu / fast::kXcd * fast::kXcd);
+    const unsigned grid = pass_grid(p, nt, z_inverse_persist<T, N>());
     hipLaunchKernelGGL((k_uxw_z_inverse<T, N>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g, tb, twz,
                        part, nt);
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
