@@ -187,6 +187,8 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
   {
     const char* env = std::getenv("GBG_GENERIC_FFT");
     p->generic_only = (env && env[0] == '1') ? 1 : 0;
+    const char* wide = std::getenv("GBG_WIDE_OFFSETS");
+    p->wide_offsets = (wide && wide[0] == '1') ? 1 : 0;
   }
   for (int a = 0; a < 3; ++a) {
     p->use_fast[a] = 1;

@@ -104,6 +104,7 @@ struct gbg_plan {
   int device;
   int ncu;           // compute units of that device
   int generic_only;  // GBG_GENERIC_FFT=1 at creation: mixed-radix LDS passes only
+  int wide_offsets;  // GBG_WIDE_OFFSETS=1 at creation: 64-bit offsets in u x omega K1
   gbg::AxisPlan ax[3];
   // per-axis transform choice (set by gbg_plan_autotune): use_fast = take the
   // register-resident pass where the length allows it; bsel = pencils per

@@ -117,18 +117,22 @@ __host__ __device__ __forceinline__ int kyc_of(int ky, const Geo& g) {
 // Tiles: B consecutive kx of one compact (kyc) row, pencil index
 // p1 = kyc * nhp + kx.  Energy and enstrophy partials land in part[2 b],
 // part[2 b + 1] for the launched workgroup b (zeros for idle ones).
-// Persistent form of K1 (see k_uxw_z_inverse): FP64 at the lengths where the
-// tile loop holds in 256 VGPRs without spilling.
-template <typename T, int N>
+// Persistent form of K1 (see k_uxw_z_inverse): where the tile loop holds in
+// the register budget without spilling (256 VGPRs in FP64; 128 in FP32, where
+// only the 32-bit-offset form at Nz = 512 fits).
+template <typename T, int N, bool O32>
 constexpr bool z_inverse_persist() {
-  return sizeof(T) == 8 && N >= 128 && N <= 512;
+  return sizeof(T) == 8 ? (N >= 128 && N <= 512) : (O32 && N == 512);
 }
 
-template <typename T, int N>
+// O32: every element offset inside one component of uh and w1 fits 32 bits
+// (M and s1 below 2^32); the per-entry offsets then take one register each.
+template <typename T, int N, bool O32>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ w1, Geo g, Tables tb,
     const cpx<T>* __restrict__ tw, double* __restrict__ part, long long ntiles) {
   using C = Cfg<T, N>;
+  using Off = typename std::conditional<O32, unsigned, long long>::type;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   __shared__ double red[2][C::NT / 64];
@@ -201,21 +205,28 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     constexpr bool ALL_STORE = 16 % B == 0;
     const bool st_ok = ALL_STORE || p1 < (long long)g.nyc * g.nhp;
     cpx<T> X[E], Y[E];
+    // The offsets pass through an opaque (volatile) copy at each use: the
+    // optimiser neither splits a load by cases nor keeps eight 64-bit
+    // addresses live across the tile, which spilled in FP32.
     auto loadc = [&](cpx<T> (&r)[E], int c) {
-      const cpx<T>* u = uh + c * g.M + off;
+      const cpx<T>* u = uh + c * g.M;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        long long zi = keep[m] ? (long long)(t + TP * m) * plane : 0;
-        asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+        Off zi = (Off)off + (keep[m] ? (Off)(t + TP * m) * (Off)plane : (Off)0);
+        asm volatile("" : "+v"(zi));
         r[m] = u[zi];
       }
     };
     auto emit = [&](cpx<T> (&v)[E], int field) {
       fast::fft<T, N, E, 1, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
       if (st_ok) {
-        cpx<T>* o = w1 + field * g.s1 + p1;
+        cpx<T>* o = w1 + field * g.s1;
 #pragma unroll
-        for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * zs] = v[m];
+        for (int m = 0; m < E; ++m) {
+          Off zo = (Off)p1 + (Off)(t + TP * m) * (Off)zs;
+          asm volatile("" : "+v"(zo));
+          o[zo] = v[m];
+        }
       }
       asm volatile("" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
@@ -288,7 +299,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     eacc += ea;
     zacc += za;
   };
-  constexpr bool PERSIST = z_inverse_persist<T, N>();
+  constexpr bool PERSIST = z_inverse_persist<T, N, O32>();
   if constexpr (PERSIST) {
     const long long per = (ntiles + fast::kXcd - 1) / fast::kXcd;
     const int xcd = blockIdx.x % fast::kXcd;
@@ -1057,9 +1068,16 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
-    const unsigned grid = pass_grid(p, nt, z_inverse_persist<T, N>());
-    hipLaunchKernelGGL((k_uxw_z_inverse<T, N>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g, tb, twz,
-                       part, nt);
+    unsigned grid;
+    if (!p->wide_offsets && g.M < (1LL << 32) && g.s1 < (1LL << 32)) {
+      grid = pass_grid(p, nt, z_inverse_persist<T, N, true>());
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
+                         tb, twz, part, nt);
+    } else {
+      grid = pass_grid(p, nt, z_inverse_persist<T, N, false>());
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
+                         tb, twz, part, nt);
+    }
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
   });
   if ((st = check_launch("uxw K1"))) return st;

@@ -229,6 +229,38 @@ def test_uxw_parity(G, n, kc, dtype, fused):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 256), (64, 64, 512), (32, 32, 1024)])
+def test_uxw_wide_offsets(G, n, dtype, monkeypatch):
+    """The z-inverse pass of u x omega addresses a component with 32-bit element
+    offsets when every component holds fewer than 2^32 modes, and with 64-bit
+    ones otherwise.  GBG_WIDE_OFFSETS=1 at plan creation forces the 64-bit form
+    at a size that fits: it must match the oracle and the 32-bit form."""
+    say(f"[uxw-wide] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    narrow = G.FieldTransformPlans(grid, dtype)
+    monkeypatch.setenv("GBG_WIDE_OFFSETS", "1")
+    wide = G.FieldTransformPlans(grid, dtype)
+    monkeypatch.delenv("GBG_WIDE_OFFSETS")
+    u = O.synthetic_field(g, 2002, vector=True, m=m)
+    kc = O.dealias_cutoff(n[0])
+    ref, E, Z = O.uxw(u, g, kc, m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    tol = TOL[dtype]
+    outs = []
+    for plans in (narrow, wide):
+        dn = G.Field(grid, "spectral", 1, dtype)
+        e, z = G.uxw_(dn, du, plans, kc, fused=True)
+        got = dn.download()
+        assert O.rel_l2(got, ref) < tol
+        assert abs(e - E) <= tol * abs(E) and abs(z - Z) <= tol * abs(Z)
+        outs.append((got, e, z))
+    assert O.rel_l2(outs[0][0], outs[1][0]) < tol
+    assert abs(outs[0][1] - outs[1][1]) <= tol * abs(E) and abs(outs[0][2] - outs[1][2]) <= tol * abs(Z)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_config1_pipeline(G, dtype):
     """64^3 plain method: rfft of a and b, gradient of a, dealiased a*b, irfft."""
     say(f"[config1] dtype={np.dtype(dtype).name}")
