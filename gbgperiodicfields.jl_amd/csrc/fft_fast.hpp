@@ -215,9 +215,15 @@ struct Cfg {
 };
 
 
-// Row-pair helpers of the x passes.  G provides nh (stored kx count of the
-// full grid), nhk (kx indices actually held, <= nh) and nyq_x (the even-axis
-// Nyquist index, or -1).
+// Row-pair helpers of the x passes, called with N = Nx.  G provides nhk (kx
+// indices actually held, <= Nx/2 + 1).  The stored count Nx/2 + 1 and the
+// Nyquist index Nx/2 follow from N at compile time, so with the element
+// index k = t + XTP m most of the completion's cases (mirrored or not, self-
+// conjugate or not) are fixed per m and cost no instruction.
+template <int N>
+__device__ __forceinline__ constexpr int half_count() { return N / 2 + 1; }
+template <int N>
+__device__ __forceinline__ constexpr int nyquist_x() { return N % 2 == 0 ? N / 2 : -1; }
 //
 // c2r_pair_load: Z_k = A_k + i B_k over the Hermitian completion of the two
 // half-spectrum rows ra, rb (entries kx >= nhk are zero), with the
@@ -234,7 +240,7 @@ __device__ __forceinline__ void c2r_pair_fetch(cpx<T> (&a)[E], cpx<T> (&b)[E],
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + C::XTP * m;
-    const int kk = k >= g.nh ? N - k : k;
+    const int kk = k >= half_count<N>() ? N - k : k;
     a[m] = mk<T>(T(0), T(0));
     b[m] = mk<T>(T(0), T(0));
     if (kk < g.nhk) {
@@ -251,10 +257,10 @@ __device__ __forceinline__ void c2r_pair_combine(cpx<T> (&v)[E], const cpx<T> (&
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + C::XTP * m;
-    const bool up = k >= g.nh;
+    const bool up = k >= half_count<N>();
     const int kk = up ? N - k : k;
     cpx<T> x = a[m], y = b[m];
-    if (kk == 0 || kk == g.nyq_x) {
+    if (kk == 0 || kk == nyquist_x<N>()) {
       x.y = T(0);
       y.y = T(0);
     }

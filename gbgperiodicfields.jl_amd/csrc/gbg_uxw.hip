@@ -642,7 +642,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int k = t + TP * m;
-        const int kk = k >= g.nh ? N - k : k;
+        const int kk = k >= fast::half_count<N>() ? N - k : k;
         a[m] = mk<T>(T(0), T(0));
         b[m] = mk<T>(T(0), T(0));
         if (kk < g.nhk) {
