@@ -83,7 +83,7 @@ def stage_bytes(n, elem):
             "K5_z_forward_project": 6 * comp}
 
 
-KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}>",
+KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}, true>",
                    "K2_y_inverse": "k_uxw_y<{t}, {n}, 1, 6>",
                    "K3_x_product": "k_uxw_x<{t}, {n}, true, false>",
                    "K4_y_forward": "k_uxw_y<{t}, {n}, -1, 3>",

@@ -776,6 +776,14 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
 // and tiles without a kept pencil are written as zeros here.
 // NC = 3: u x omega (Leray projection, mask, mean-free); NC = 1: the scalar
 // product (mask only: its mean mode is kept).
+// Persistent form (one workgroup per CU walking its XCD's share of the
+// tiles): FP64 at the lengths where the tile loop holds in 256 VGPRs without
+// spilling.  In FP32 it spills at the 128-VGPR limit, 32-bit offsets or not.
+template <typename T, int N>
+constexpr bool z_forward_persist() {
+  return sizeof(T) == 8 && (N == 128 || N == 512);
+}
+
 template <typename T, int N, int NC>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ w1, cpx<T>* __restrict__ nh, Geo g, Tables tb,
@@ -784,35 +792,6 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
-  const long long lt = fast::xcd_tile();
-  const long long tile = (list && lt < ntiles) ? (long long)list[lt] : lt;
-  const long long p5l = tile * B + pl;
-  const bool valid = lt < ntiles && p5l < (long long)g.ny * g.nh;
-  const int p5 = valid ? (int)p5l : 0;
-  int kx = 0, ky = 0, kyc = -1;
-  double qxy = 0.0;
-  bool disk = false;
-  if (valid) {
-    ky = p5 / g.nh;
-    kx = p5 - ky * g.nh;
-    kyc = kyc_of(ky, g);
-    qxy = tb.ks1[kx] + tb.ks2[ky];
-    disk = kx < g.nhk && kyc >= 0 && qxy <= g.hi2;
-  }
-  const long long plane = (long long)g.ny * g.nh;
-  if (!__syncthreads_or(disk)) {
-    if (valid) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int m = 0; m < E; ++m) nh[c * g.M + (long long)(t + TP * m) * plane + p5] = mk<T>(T(0), T(0));
-    }
-    return;
-  }
-  const long long zs = (long long)g.nyc * g.nhp;
-  const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
-  cpx<T>* pen = lds + pl * LD;
-  cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
   // A global load consumed between the component loads and their last use
   // would wait for all of them (the counter of outstanding loads drains in
   // order): for N <= 512 (STREAM) the twiddles come from an LDS copy
@@ -831,67 +810,109 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     else
       return tw;
   };
-  bool keep[E];
+  // one tile; the barrier at its start also orders the previous tile's last
+  // LDS reads before this tile's first writes
+  auto run_tile = [&](long long lt) {
+    const long long tile = (list && lt < ntiles) ? (long long)list[lt] : lt;
+    const long long p5l = tile * B + pl;
+    const bool valid = lt < ntiles && p5l < (long long)g.ny * g.nh;
+    const int p5 = valid ? (int)p5l : 0;
+    int kx = 0, ky = 0, kyc = -1;
+    double qxy = 0.0;
+    bool disk = false;
+    if (valid) {
+      ky = p5 / g.nh;
+      kx = p5 - ky * g.nh;
+      kyc = kyc_of(ky, g);
+      qxy = tb.ks1[kx] + tb.ks2[ky];
+      disk = kx < g.nhk && kyc >= 0 && qxy <= g.hi2;
+    }
+    const long long plane = (long long)g.ny * g.nh;
+    if (!__syncthreads_or(disk)) {
+      if (valid) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int kz = t + TP * m;
-    keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) &  // no branch
-              (NC == 1 || !(kx == 0 && ky == 0 && kz == 0));
-  }
-  // STREAM: components off the disk read the compact buffer's first entry (a
-  // valid address) and are zeroed just before their transform
-  cpx<T> n[NC][E];
+        for (int c = 0; c < NC; ++c)
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const cpx<T>* in = w1 + c * g.s1 + cb;
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      if constexpr (STREAM) {
-        long long zi = disk ? (long long)(t + TP * m) * zs : 0;
-        asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
-        n[c][m] = in[zi];
-      } else {
-        n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
+          for (int m = 0; m < E; ++m) nh[c * g.M + (long long)(t + TP * m) * plane + p5] = mk<T>(T(0), T(0));
       }
+      return;
     }
-  }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if constexpr (STREAM) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) n[c][m] = keep_or_zero(n[c][m], disk);
-    }
-    fast::fft<T, N, E, -1, 0, C::PD, C::ONE>(n[c], pen, pen1, t, twp());
-    if constexpr (STREAM) __builtin_amdgcn_sched_barrier(0);
-  }
-  if (!valid) return;
-  if constexpr (NC == 1) {
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const long long o = (long long)(t + TP * m) * plane + p5;
-      nh[o] = mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0));
-    }
-  } else {
-    const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+    const long long zs = (long long)g.nyc * g.nhp;
+    const long long cb = (long long)(disk ? kyc : 0) * g.nhp + (disk ? kx : 0);
+    cpx<T>* pen = lds + pl * LD;
+    cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
+    bool keep[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int kz = t + TP * m;
-      const T a3 = T(tb.kd3[kz]);
-      // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
-      const T sq = a1 * a1 + a2 * a2 + a3 * a3;
-      const T den = sq == T(0) ? T(1) : sq;
-      const cpx<T> dot = cadd(cadd(cscale(n[0][m], a1), cscale(n[1][m], a2)), cscale(n[2][m], a3));
-      const cpx<T> lon = mk<T>(dot.x / den, dot.y / den);
-      const cpx<T> p1 = csub(n[0][m], cscale(lon, a1));
-      const cpx<T> p2 = csub(n[1][m], cscale(lon, a2));
-      const cpx<T> p3 = csub(n[2][m], cscale(lon, a3));
-      const long long o = (long long)kz * plane + p5;
-      // component-wise selects (a select of whole complex values is lowered
-      // through a private array in FP64)
-      nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
-      nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
-      nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+      keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) &  // no branch
+                (NC == 1 || !(kx == 0 && ky == 0 && kz == 0));
     }
+    // STREAM: components off the disk read the compact buffer's first entry (a
+    // valid address) and are zeroed just before their transform
+    cpx<T> n[NC][E];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const cpx<T>* in = w1 + c * g.s1 + cb;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        if constexpr (STREAM) {
+          long long zi = disk ? (long long)(t + TP * m) * zs : 0;
+          asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
+          n[c][m] = in[zi];
+        } else {
+          n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if constexpr (STREAM) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) n[c][m] = keep_or_zero(n[c][m], disk);
+      }
+      fast::fft<T, N, E, -1, 0, C::PD, C::ONE>(n[c], pen, pen1, t, twp());
+      if constexpr (STREAM) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (!valid) return;
+    if constexpr (NC == 1) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const long long o = (long long)(t + TP * m) * plane + p5;
+        nh[o] = mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0));
+      }
+    } else {
+      const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int kz = t + TP * m;
+        const T a3 = T(tb.kd3[kz]);
+        // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
+        const T sq = a1 * a1 + a2 * a2 + a3 * a3;
+        const T den = sq == T(0) ? T(1) : sq;
+        const cpx<T> dot = cadd(cadd(cscale(n[0][m], a1), cscale(n[1][m], a2)), cscale(n[2][m], a3));
+        const cpx<T> lon = mk<T>(dot.x / den, dot.y / den);
+        const cpx<T> p1 = csub(n[0][m], cscale(lon, a1));
+        const cpx<T> p2 = csub(n[1][m], cscale(lon, a2));
+        const cpx<T> p3 = csub(n[2][m], cscale(lon, a3));
+        const long long o = (long long)kz * plane + p5;
+        // component-wise selects (a select of whole complex values is lowered
+        // through a private array in FP64)
+        nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
+        nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
+        nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+      }
+    }
+  };
+  if constexpr (z_forward_persist<T, N>()) {
+    const long long per = (ntiles + fast::kXcd - 1) / fast::kXcd;
+    const int xcd = blockIdx.x % fast::kXcd;
+    const long long L = gridDim.x / fast::kXcd;
+    const long long tend = (xcd + 1) * per < ntiles ? (xcd + 1) * per : ntiles;
+#pragma unroll 1
+    for (long long lt = xcd * per + blockIdx.x / fast::kXcd; lt < tend; lt += L) run_tile(lt);
+  } else {
+    run_tile(fast::xcd_tile());
   }
 }
 
@@ -933,6 +954,23 @@ static unsigned pass_grid(const gbg_plan* p, long long nt, bool persist) {
   const unsigned cap = (unsigned)(p->ncu / fast::kXcd * fast::kXcd);
   if (persist && cap >= (unsigned)fast::kXcd && grid > cap) grid = cap;
   return grid;
+}
+
+// 32-bit element offsets inside one component of the u x omega buffers (the
+// input and output spectra, M modes each, and the compact intermediates, s1
+// each); GBG_WIDE_OFFSETS=1 at plan creation forces the 64-bit form
+static bool narrow_offsets(const gbg_plan* p, const Geo& g) {
+  return !p->wide_offsets && g.M < (1LL << 32) && g.s1 < (1LL << 32);
+}
+
+// K5's transform kernel over nt tiles (tile list `list`, or all tiles)
+template <typename T, int N, int NC>
+static void launch_z_forward(const gbg_plan* p, const Geo& g, long long nt, const cpx<T>* w,
+                             cpx<T>* out, const Tables& tb, const cpx<T>* twz, const int* list,
+                             hipStream_t s) {
+  using C = Cfg<T, N>;
+  hipLaunchKernelGGL((k_uxw_z_forward<T, N, NC>), dim3(pass_grid(p, nt, z_forward_persist<T, N>())),
+                     dim3(C::NT), 0, s, w, out, g, tb, twz, list, nt);
 }
 
 static bool fast_len(int n) {
@@ -1069,7 +1107,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     using C = Cfg<T, N>;
     const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
     unsigned grid;
-    if (!p->wide_offsets && g.M < (1LL << 32) && g.s1 < (1LL << 32)) {
+    if (narrow_offsets(p, g)) {
       grid = pass_grid(p, nt, z_inverse_persist<T, N, true>());
       hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
                          tb, twz, part, nt);
@@ -1123,8 +1161,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     const TileSet* ts = tile_set(p, g, C::B, kc, s);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 3>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W2,
-                         out, g, tb, twz, (const int*)nullptr, nt);
+      launch_z_forward<T, N, 3>(p, g, nt, W2, out, tb, twz, nullptr, s);
       return;
     }
     if (ts->nd < ts->nt) {
@@ -1136,8 +1173,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
                          plane, np);
     }
     if (ts->nd > 0)
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 3>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0, s,
-                         W2, out, g, tb, twz, (const int*)ts->dev, (long long)ts->nd);
+      launch_z_forward<T, N, 3>(p, g, ts->nd, W2, out, tb, twz, ts->dev, s);
   });
   if ((st = check_launch("uxw K5"))) return st;
   prof_mark(p, s);
@@ -1241,8 +1277,7 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
     const TileSet* ts = tile_set(p, g, C::B, kc, s);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s,
-                         W2, out, g, tb, twz, (const int*)nullptr, nt);
+      launch_z_forward<T, N, 1>(p, g, nt, W2, out, tb, twz, nullptr, s);
       return;
     }
     if (ts->nd < ts->nt) {
@@ -1252,8 +1287,7 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
                          plane, g.nz);
     }
     if (ts->nd > 0)
-      hipLaunchKernelGGL((k_uxw_z_forward<T, N, 1>), dim3(fast::xcd_grid(ts->nd)), dim3(C::NT), 0,
-                         s, W2, out, g, tb, twz, (const int*)ts->dev, (long long)ts->nd);
+      launch_z_forward<T, N, 1>(p, g, ts->nd, W2, out, tb, twz, ts->dev, s);
   });
   if ((st = check_launch("product P5"))) return st;
   prof_mark(p, s);
