@@ -261,6 +261,42 @@ def test_uxw_wide_offsets(G, n, dtype, monkeypatch):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(32, 32, 128), (16, 16, 512)])
+def test_many_cutoffs(G, n, dtype):
+    """A plan caches the output tile list of up to 8 cutoffs; further cutoffs
+    run the last z pass over every tile (zeros written in-kernel), persistent
+    in FP64 at these lengths.  Both forms, u x omega and the scalar product,
+    against the oracle for ten cutoffs on one plan."""
+    say(f"[many-cutoffs] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = O.synthetic_field(g, 2003, vector=True, m=m)
+    a_hat = O.synthetic_field(g, 1003, m=m)
+    b_hat = O.synthetic_field(g, 1004, m=m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    ah = G.Field(grid, "spectral", 0, dtype).upload(a_hat)
+    bh = G.Field(grid, "spectral", 0, dtype).upload(b_hat)
+    tol = TOL[dtype]
+    for kc in (3.5, 4.5, 5.5, 6.5, 7.5, 8.5, 9.5, 10.5, 11.5, 12.5):
+        ref, E, Z = O.uxw(u, g, kc, m)
+        dn = G.Field(grid, "spectral", 1, dtype)
+        e, z = G.uxw_(dn, du, plans, kc, fused=True)
+        got = dn.download()
+        assert O.rel_l2(got, ref) < tol, kc
+        assert abs(e - E) <= tol * abs(E) and abs(z - Z) <= tol * abs(Z)
+        mask = O.band_mask(m, 0.0, kc)
+        assert np.all(got[:, mask == 0] == 0)
+        pref = O.dealiased_product(a_hat, b_hat, g, kc, m)
+        ph = G.Field(grid, "spectral", 0, dtype)
+        G.dealiased_product_(ph, ah, bh, plans, kc)
+        pg = ph.download()
+        assert O.rel_l2(pg, pref) < tol, kc
+        assert np.all(pg[mask == 0] == 0)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_config1_pipeline(G, dtype):
     """64^3 plain method: rfft of a and b, gradient of a, dealiased a*b, irfft."""
     say(f"[config1] dtype={np.dtype(dtype).name}")
