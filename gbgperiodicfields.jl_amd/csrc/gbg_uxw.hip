@@ -18,9 +18,14 @@
 //   K5  z-forward:   3 fields -> Leray projection, mask, mean-free -> N^ (full
 //                    layout, zeros outside the mask), on the output tiles that
 //                    hold disk pencils; a write-only kernel zeros the other tiles
-// Buffers: W1 = [6][Nz][nyc][nhk] (K1 -> K2), reused as [3][Nz][nyc][nhk]
-// (K4 -> K5); W2 = [6][Nz][Ny][nhk] (K2 -> K3), reused as [3][Nz][Ny][nhk]
-// (K3 -> K4).  nhk = kx_max + 1, nyc = number of kept ky indices.
+// Buffers: W1 = [6][Nz][nyc][nhp] (K1 -> K2), reused as [3][Nz][nyc][nhp]
+// (K4 -> K5); W2 = [6][Nz][Ny][nhp] (K2 -> K3), reused as [3][Nz][Ny][nhp]
+// (K3 -> K4).  nhk = kx_max + 1 entries of each row are used, nhp = nhk
+// rounded up to 16; nyc = number of kept ky indices.
+// The z passes (K1, K5's transform kernel) have one 131-160 KB-LDS workgroup
+// per CU; where their tile loop fits the register budget without spilling
+// they run persistent (one workgroup per CU walking its XCD's tiles), so the
+// CU does not sit empty between one tile's workgroup and the next.
 #include <type_traits>
 #include <vector>
 
@@ -685,12 +690,13 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
     return;
   }
   T N3[E], n3a[E];
-  // EARLY (N <= 512): the raw entries na, nb of the second row's first field
-  // pair are fetched before the first row's output stores are issued -- the
-  // staging wait covers every outstanding load and store, so fetching them
-  // after the stores would also wait for the stores to complete -- and the
-  // two rows are written out.  Where that form spills (FP64 at N = 32, 256
-  // and 1024, FP32 at 1024), the rows loop and each fetches its first pair.
+  // EARLY: the raw entries na, nb of the second row's first field pair are
+  // fetched before the first row's output stores are issued -- the staging
+  // wait covers every outstanding load and store, so fetching them after the
+  // stores would also wait for the stores to complete -- and the two rows are
+  // written out.  Elsewhere the rows loop and each fetches its first pair: at
+  // Nx = 1024 without staging the written-out form spills, and where it fits
+  // otherwise (FP64 at 32 and 256, FP32 at 1024) it measured no faster.
   constexpr bool EARLY = sizeof(T) == 4 ? N <= 512 : (N == 64 || N == 128 || N == 512);
   cpx<T> na[E], nb[E];
   issue(ro0, 0);
@@ -778,7 +784,8 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __r
 // product (mask only: its mean mode is kept).
 // Persistent form (one workgroup per CU walking its XCD's share of the
 // tiles): FP64 at the lengths where the tile loop holds in 256 VGPRs without
-// spilling.  In FP32 it spills at the 128-VGPR limit, 32-bit offsets or not.
+// spilling.  In FP32 the tile loop spills at the 128-VGPR limit (also with
+// 32-bit element offsets as in K1).
 template <typename T, int N>
 constexpr bool z_forward_persist() {
   return sizeof(T) == 8 && (N == 128 || N == 512);
