@@ -9,7 +9,8 @@ module GBGHipExt
 using GBGPeriodicFields
 import GBGPeriodicFields: to_spectral!, to_physical!, gradient!, divergence!, curl!,
     inverse_curl!, laplacian!, inverse_laplacian!, helmholtz_projection!, convolve!,
-    mean_free_projection!, cross!, multiply!, inner
+    mean_free_projection!, cross!, multiply!, inner, l2norm, exponential_convolve!,
+    radial_spectrum, reality_residual, reality_projection!
 
 const libgbg = get(ENV, "GBG_LIB", joinpath(@__DIR__, "..", "libgbg.so"))
 
@@ -100,6 +101,13 @@ divergence!(dst, src, plan::GBGPlan) =
                   Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), field_tensor_order(src), 0, 0.0, 0.0, 1.0, 0.0,
                  C_NULL)); dst)
+# divergence!(dst, src, m, spectral_band(m, klo, khi), α, β): band kernel and α/β fused
+divergence!(dst, src, plan::GBGPlan, klo, khi, α = 1.0, β = 0.0) =
+    (check(ccall((:gbg_divergence, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Cdouble, Cdouble, Cdouble, Cdouble,
+                  Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(src), field_tensor_order(src), 1, klo, khi, α, β,
+                 C_NULL)); dst)
 curl!(dst, src, plan::GBGPlan) =
     (check(ccall((:gbg_curl, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), C_NULL)); dst)
@@ -117,6 +125,11 @@ helmholtz_projection!(dst, src, plan::GBGPlan) =
     (check(ccall((:gbg_helmholtz, libgbg), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cvoid}),
                  plan.handle, dptr(dst), dptr(src), 0, 0.0, 0.0, 1.0, 0.0, C_NULL)); dst)
+# helmholtz_projection!(dst, src, m, spectral_band(m, klo, khi), α, β) (:1589-1593)
+helmholtz_projection!(dst, src, plan::GBGPlan, klo, khi, α = 1.0, β = 0.0) =
+    (check(ccall((:gbg_helmholtz, libgbg), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cdouble, Cdouble, Cdouble, Cdouble, Ptr{Cvoid}),
+                 plan.handle, dptr(dst), dptr(src), 1, klo, khi, α, β, C_NULL)); dst)
 # convolve!(dst, src, spectral_cutoff(m, kc)) -> band [0, kc]
 convolve_band!(dst, src, plan::GBGPlan, klo, khi, α = 1.0, β = 0.0) =
     (check(ccall((:gbg_convolve_band, libgbg), Cint,
@@ -145,6 +158,9 @@ function inner(f::AbstractFieldSpectral, g::AbstractFieldSpectral, plan::GBGPlan
                 plan.handle, dptr(f), dptr(g), ncomp(f), out, C_NULL))
     return out[]
 end
+
+# l2norm(f) = sqrt(inner(f, f)) (src/Reductions.jl:947-948), either representation
+l2norm(f, plan::GBGPlan) = sqrt(inner(f, f, plan))
 
 # ---- exponential kernel (src/Convolution.jl:1750) ----------------------------------------
 optr(offset) = offset === nothing ? C_NULL : dptr(offset)
