@@ -152,6 +152,9 @@ static void destroy_plan(gbg_plan* p) {
     if (p->tset[i].dev) (void)hipFree(p->tset[i].dev);
   for (int i = 0; i < kMaxEvents; ++i)
     if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+  if (p->aux_fork) (void)hipEventDestroy(p->aux_fork);
+  if (p->aux_join) (void)hipEventDestroy(p->aux_join);
+  if (p->aux) (void)hipStreamDestroy(p->aux);
   delete p;
 }
 

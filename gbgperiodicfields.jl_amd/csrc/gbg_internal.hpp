@@ -126,6 +126,10 @@ struct gbg_plan {
   gbg::TileSet tset[gbg::kTileSets];
   int ntset;
   size_t device_bytes;
+  // second stream of the fused pipelines and its fork / join events (created
+  // on first use outside a graph capture)
+  hipStream_t aux;
+  hipEvent_t aux_fork, aux_join;
   // profiling events
   int profiling;
   int nev;

@@ -434,3 +434,9 @@ def test_refusals(G):
         G.curl_(s, s, other)
     with pytest.raises(G.UnsupportedError):
         G.curl_(G.Field(grid, "physical", 1), s, plans)
+    # the fused pipelines refuse an output that overlaps an input
+    with pytest.raises(G.AliasError):
+        G.uxw_(s, s, plans)
+    a = G.Field(grid, "spectral", 0)
+    with pytest.raises(G.AliasError):
+        G.dealiased_product_(a, a, G.Field(grid, "spectral", 0), plans)
