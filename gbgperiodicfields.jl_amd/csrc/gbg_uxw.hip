@@ -1096,11 +1096,13 @@ static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipS
 // (zero_tiles_join); 512^3 u x omega FP32: 2.92 -> 2.86 ms.  In FP64 the
 // forked form measured 0.5 % slower (the z-inverse pass leaves no room for
 // the zero writes beside it), so there, and whenever the fork is not taken,
-// the last pass launches them on s itself.
+// the last pass launches them on s itself.  Nor is the fork taken while s is
+// being captured into a graph: the batched form's graph of many small items
+// ran 1.6x longer in FP32 with a fork and join per item.
 static bool aux_stream(gbg_plan* p, hipStream_t s) {
-  if (p->aux) return true;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+  if (p->aux) return true;
   hipStream_t a = nullptr;
   hipEvent_t f = nullptr, j = nullptr;
   if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess &&
