@@ -211,6 +211,13 @@ __global__ __launch_bounds__(kOpNT) void k_multiply(T* dst, const T* a, const T*
 // ---- Parseval inner, FP64 accumulation in a fixed order --------------------------------
 // sum_c sum_modes w(kx) Re(f conj g), w = 1 on kx = 0 and the even Nyquist plane,
 // else 2; result / N^2   (src/Reductions.jl:883-925, weight :956-965)
+// Each block sums a fixed contiguous chunk; a thread takes every kOpNT-th entry
+// of it, kRedU loads in flight at a time into four accumulators (a single
+// accumulator made each step wait for the previous add and left the loads
+// unoverlapped).  The stored length M is a multiple of nh, so kx = i mod nh
+// over the flattened (component, mode) index and advances by kOpNT mod nh.
+constexpr int kRedU = 8;
+
 template <typename T>
 __global__ __launch_bounds__(kOpNT) void k_inner_part(const cpx<T>* f, const cpx<T>* g, size_t M,
                                                       int ncomp, unsigned nh, int nyq,
@@ -220,15 +227,28 @@ __global__ __launch_bounds__(kOpNT) void k_inner_part(const cpx<T>* f, const cpx
   const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
   const size_t lo = chunk * blockIdx.x;
   const size_t hi = lo + chunk < total ? lo + chunk : total;
-  double acc = 0.0;
-  for (size_t i = lo + threadIdx.x; i < hi; i += kOpNT) {
-    const size_t m = i % M;
-    const unsigned kx = (unsigned)(m % nh);
+  const unsigned step = kOpNT % nh;
+  size_t i = lo + threadIdx.x;
+  unsigned kx = (unsigned)(i % nh);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  auto add = [&](int u, const cpx<T>& a, const cpx<T>& b) {
     const double w = (kx == 0 || (int)kx == nyq) ? 1.0 : 2.0;
-    const cpx<T> a = f[i], b = g[i];
-    acc += w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y);
+    acc[u & 3] += w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y);
+    kx += step;
+    if (kx >= nh) kx -= nh;
+  };
+  for (; i + (size_t)(kRedU - 1) * kOpNT < hi; i += (size_t)kRedU * kOpNT) {
+    cpx<T> a[kRedU], b[kRedU];
+#pragma unroll
+    for (int u = 0; u < kRedU; ++u) {
+      a[u] = f[i + (size_t)u * kOpNT];
+      b[u] = g[i + (size_t)u * kOpNT];
+    }
+#pragma unroll
+    for (int u = 0; u < kRedU; ++u) add(u, a[u], b[u]);
   }
-  red[threadIdx.x] = acc;
+  for (; i < hi; i += kOpNT) add(0, f[i], g[i]);
+  red[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   for (int s = kOpNT / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -246,9 +266,20 @@ __global__ __launch_bounds__(kOpNT) void k_dot_part(const T* a, const T* b, size
   const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
   const size_t lo = chunk * blockIdx.x;
   const size_t hi = lo + chunk < total ? lo + chunk : total;
-  double acc = 0.0;
-  for (size_t i = lo + threadIdx.x; i < hi; i += kOpNT) acc += (double)a[i] * (double)b[i];
-  red[threadIdx.x] = acc;
+  size_t i = lo + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (; i + (size_t)(kRedU - 1) * kOpNT < hi; i += (size_t)kRedU * kOpNT) {
+    T x[kRedU], y[kRedU];
+#pragma unroll
+    for (int u = 0; u < kRedU; ++u) {
+      x[u] = a[i + (size_t)u * kOpNT];
+      y[u] = b[i + (size_t)u * kOpNT];
+    }
+#pragma unroll
+    for (int u = 0; u < kRedU; ++u) acc[u & 3] += (double)x[u] * (double)y[u];
+  }
+  for (; i < hi; i += kOpNT) acc[0] += (double)a[i] * (double)b[i];
+  red[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   for (int s = kOpNT / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
