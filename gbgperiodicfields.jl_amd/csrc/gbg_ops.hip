@@ -11,15 +11,25 @@ namespace gbg {
 
 constexpr int kOpNT = 256;
 
-// decode a linear mode index into (kx, ky, kz)
+// decode a linear mode index into (kx, ky, kz).  The two divisions by the
+// run-time nh and ny go through an FP64 reciprocal: n rh (n < 2^32, rh = 1/nh
+// rounded) is within 2^-21 of n / nh, so its truncation is the quotient or,
+// when n / nh is an integer, possibly one less, which the remainder test
+// corrects.  Seven instructions instead of an integer division's thirty.
 struct ModeIdx {
   unsigned nh, ny;
+  double rh, ry;
+  __device__ __forceinline__ static unsigned div(unsigned n, unsigned d, double rd) {
+    unsigned q = (unsigned)((double)n * rd);
+    q += (n - q * d) >= d ? 1u : 0u;
+    return q;
+  }
   __device__ __forceinline__ void operator()(unsigned idx, unsigned& kx, unsigned& ky,
                                              unsigned& kz) const {
-    kx = idx % nh;
-    const unsigned t = idx / nh;
-    ky = t % ny;
-    kz = t / ny;
+    const unsigned t = div(idx, nh, rh);
+    kx = idx - t * nh;
+    kz = div(t, ny, ry);
+    ky = t - kz * ny;
   }
 };
 
@@ -31,6 +41,20 @@ __device__ __forceinline__ bool in_band(const Tables& t, unsigned kx, unsigned k
 }
 
 static unsigned nblocks(size_t n) { return (unsigned)((n + kOpNT - 1) / kOpNT); }
+
+// Kernels that move few bytes per entry (the Laplacians of a scalar, the scalar
+// product) take kSpan entries per thread, every load issued before the first
+// store: with one entry per thread too few bytes were in flight to cover the
+// HBM latency.  Entries past the end load the last entry (a valid address)
+// and are not stored.
+// Only FP32 takes the spans: in FP64 (16-byte modes) one entry per thread
+// measured faster for the vector Laplacian.
+template <typename T>
+constexpr int span() { return sizeof(T) == 4 ? 4 : 1; }
+template <typename T>
+static unsigned nblocks_span(size_t n) {
+  return (unsigned)((n + (size_t)kOpNT * span<T>() - 1) / ((size_t)kOpNT * span<T>()));
+}
 
 // ---- gradient ----------------------------------------------------------------
 // scalar -> vector: d_j = (i a_j) s      (_scalar_gradient_kernel!, :357-368)
@@ -118,15 +142,29 @@ __global__ __launch_bounds__(kOpNT) void k_curl(cpx<T>* dst, const cpx<T>* src, 
 template <typename T, bool INV>
 __global__ __launch_bounds__(kOpNT) void k_laplacian(cpx<T>* dst, const cpx<T>* src, unsigned M,
                                                      int ncomp, ModeIdx mi, Tables t) {
-  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
-  if (idx >= M) return;
-  unsigned kx, ky, kz;
-  mi(idx, kx, ky, kz);
-  const T q = (T(t.ks1[kx]) + T(t.ks2[ky])) + T(t.ks3[kz]);
-  const T f = INV ? (q == T(0) ? T(0) : T(1) / q) : -q;
+  constexpr int kSpan = span<T>();
+  const unsigned base = blockIdx.x * (kOpNT * kSpan) + threadIdx.x;
+  unsigned at[kSpan];
+  T f[kSpan];
+#pragma unroll
+  for (int j = 0; j < kSpan; ++j) {
+    const unsigned idx = base + j * kOpNT;
+    at[j] = idx < M ? idx : M - 1;
+    unsigned kx, ky, kz;
+    mi(at[j], kx, ky, kz);
+    const T q = (T(t.ks1[kx]) + T(t.ks2[ky])) + T(t.ks3[kz]);
+    f[j] = INV ? (q == T(0) ? T(0) : T(1) / q) : -q;
+  }
   for (int c = 0; c < ncomp; ++c) {
-    const cpx<T> v = src[(size_t)c * M + idx];
-    dst[(size_t)c * M + idx] = INV ? cscale(mk<T>(-v.x, -v.y), f) : cscale(v, f);
+    cpx<T> v[kSpan];
+#pragma unroll
+    for (int j = 0; j < kSpan; ++j) v[j] = src[(size_t)c * M + at[j]];
+#pragma unroll
+    for (int j = 0; j < kSpan; ++j) {
+      const unsigned idx = base + j * kOpNT;
+      if (idx < M)
+        dst[(size_t)c * M + idx] = INV ? cscale(mk<T>(-v[j].x, -v[j].y), f[j]) : cscale(v[j], f[j]);
+    }
   }
 }
 
@@ -203,9 +241,21 @@ __global__ __launch_bounds__(kOpNT) void k_cross(T* dst, const T* u, const T* v,
 
 template <typename T>
 __global__ __launch_bounds__(kOpNT) void k_multiply(T* dst, const T* a, const T* b, size_t P) {
-  const size_t i = (size_t)blockIdx.x * kOpNT + threadIdx.x;
-  if (i >= P) return;
-  dst[i] = a[i] * b[i];
+  constexpr int kSpan = span<T>();
+  const size_t base = (size_t)blockIdx.x * (kOpNT * kSpan) + threadIdx.x;
+  T x[kSpan], y[kSpan];
+#pragma unroll
+  for (int j = 0; j < kSpan; ++j) {
+    const size_t i = base + (size_t)j * kOpNT;
+    const size_t ic = i < P ? i : P - 1;
+    x[j] = a[ic];
+    y[j] = b[ic];
+  }
+#pragma unroll
+  for (int j = 0; j < kSpan; ++j) {
+    const size_t i = base + (size_t)j * kOpNT;
+    if (i < P) dst[i] = x[j] * y[j];
+  }
 }
 
 // ---- Parseval inner, FP64 accumulation in a fixed order --------------------------------
@@ -305,7 +355,9 @@ __global__ __launch_bounds__(kOpNT) void k_inner_final(const double* part, int n
 // ----------------------------------------------------------------------------------------
 // host launchers
 // ----------------------------------------------------------------------------------------
-static ModeIdx modes_of(gbg_plan* p) { return ModeIdx{(unsigned)p->nh, (unsigned)p->n[1]}; }
+static ModeIdx modes_of(gbg_plan* p) {
+  return ModeIdx{(unsigned)p->nh, (unsigned)p->n[1], 1.0 / (double)p->nh, 1.0 / (double)p->n[1]};
+}
 
 gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, hipStream_t s) {
   const unsigned M = (unsigned)p->M;
@@ -360,17 +412,17 @@ gbg_status op_laplacian(gbg_plan* p, void* dst, const void* src, int ncomp, int 
   const unsigned M = (unsigned)p->M;
   if (p->elem == 8) {
     if (inverse)
-      hipLaunchKernelGGL((k_laplacian<double, true>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+      hipLaunchKernelGGL((k_laplacian<double, true>), nblocks_span<double>(M), kOpNT, 0, s, (cpx<double>*)dst,
                          (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p));
     else
-      hipLaunchKernelGGL((k_laplacian<double, false>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
+      hipLaunchKernelGGL((k_laplacian<double, false>), nblocks_span<double>(M), kOpNT, 0, s, (cpx<double>*)dst,
                          (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p));
   } else {
     if (inverse)
-      hipLaunchKernelGGL((k_laplacian<float, true>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+      hipLaunchKernelGGL((k_laplacian<float, true>), nblocks_span<float>(M), kOpNT, 0, s, (cpx<float>*)dst,
                          (const cpx<float>*)src, M, ncomp, modes_of(p), tables_of(p));
     else
-      hipLaunchKernelGGL((k_laplacian<float, false>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
+      hipLaunchKernelGGL((k_laplacian<float, false>), nblocks_span<float>(M), kOpNT, 0, s, (cpx<float>*)dst,
                          (const cpx<float>*)src, M, ncomp, modes_of(p), tables_of(p));
   }
   return check_launch("laplacian");
@@ -426,10 +478,10 @@ gbg_status op_cross(gbg_plan* p, void* dst, const void* u, const void* v, hipStr
 
 gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hipStream_t s) {
   if (p->elem == 8)
-    hipLaunchKernelGGL((k_multiply<double>), nblocks(p->P), kOpNT, 0, s, (double*)dst,
+    hipLaunchKernelGGL((k_multiply<double>), nblocks_span<double>(p->P), kOpNT, 0, s, (double*)dst,
                        (const double*)a, (const double*)b, p->P);
   else
-    hipLaunchKernelGGL((k_multiply<float>), nblocks(p->P), kOpNT, 0, s, (float*)dst,
+    hipLaunchKernelGGL((k_multiply<float>), nblocks_span<float>(p->P), kOpNT, 0, s, (float*)dst,
                        (const float*)a, (const float*)b, p->P);
   return check_launch("multiply");
 }
