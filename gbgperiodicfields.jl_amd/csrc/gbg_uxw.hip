@@ -1099,9 +1099,15 @@ static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipS
 // the last pass launches them on s itself.  Nor is the fork taken while s is
 // being captured into a graph: the batched form's graph of many small items
 // ran 1.6x longer in FP32 with a fork and join per item.
+// A failed query or set-up call only declines the fork; its error is cleared
+// so that the next launch check does not report it.
 static bool aux_stream(gbg_plan* p, hipStream_t s) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (cs != hipStreamCaptureStatusNone) return false;
   if (p->aux) return true;
   hipStream_t a = nullptr;
   hipEvent_t f = nullptr, j = nullptr;
@@ -1148,7 +1154,10 @@ static const TileSet* zero_tiles_fork(gbg_plan* p, const Geo& g, double kc, int 
     constexpr int B = Cfg<T, decltype(L)::value>::B;
     const TileSet* ts = tile_set(p, g, B, kc, s);
     if (!ts || ts->nd == ts->nt || !aux_stream(p, s)) return;
-    if (hipEventRecord(p->aux_fork, s) == hipSuccess) fork = ts;
+    if (hipEventRecord(p->aux_fork, s) == hipSuccess)
+      fork = ts;
+    else
+      (void)hipGetLastError();
   });
   return fork;
 }
@@ -1158,9 +1167,19 @@ static const TileSet* zero_tiles_fork(gbg_plan* p, const Geo& g, double kc, int 
 // fill the room they leave
 template <typename T>
 static bool zero_tiles_launch(gbg_plan* p, const Geo& g, const TileSet* ts, int nc, cpx<T>* out) {
-  if (!ts || hipStreamWaitEvent(p->aux, p->aux_fork, 0) != hipSuccess) return false;
+  if (!ts) return false;
+  if (hipStreamWaitEvent(p->aux, p->aux_fork, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
   launch_zero_tiles<T>(g, ts, ts->b, nc, out, p->aux);
-  return hipEventRecord(p->aux_join, p->aux) == hipSuccess;
+  if (hipEventRecord(p->aux_join, p->aux) == hipSuccess) return true;
+  // no join point: let the second stream drain, and the last pass writes the
+  // zero tiles on s as well
+  (void)hipGetLastError();
+  (void)hipStreamSynchronize(p->aux);
+  (void)hipGetLastError();
+  return false;
 }
 
 static gbg_status zero_tiles_join(gbg_plan* p, hipStream_t s) {
