@@ -168,6 +168,58 @@ __global__ __launch_bounds__(kOpNT) void k_laplacian(cpx<T>* dst, const cpx<T>* 
   }
 }
 
+// FP32 scalar field with an even mode count and 16-byte aligned buffers: two
+// consecutive modes per lane, one 16-byte load and store each, two such pairs
+// per thread (8-byte lanes left the FP32 scalar Laplacians at about 4.6 TB/s;
+// 512^3 inverse Laplacian 0.23 -> 0.21 ms).  For a vector field it measured
+// slower (0.57 -> 0.59 ms), so three components keep k_laplacian.  The same
+// per-mode arithmetic as k_laplacian.
+template <bool INV>
+__global__ __launch_bounds__(kOpNT) void k_laplacian_pair(float4* dst, const float4* src,
+                                                          unsigned M2, int ncomp, ModeIdx mi,
+                                                          Tables t) {
+  constexpr int kPairs = 2;
+  const unsigned base = blockIdx.x * (kOpNT * kPairs) + threadIdx.x;
+  unsigned at[kPairs];
+  float f[kPairs][2];
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const unsigned idx = base + j * kOpNT;
+    at[j] = idx < M2 ? idx : M2 - 1;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      unsigned kx, ky, kz;
+      mi(2 * at[j] + h, kx, ky, kz);
+      const float q = (float(t.ks1[kx]) + float(t.ks2[ky])) + float(t.ks3[kz]);
+      f[j][h] = INV ? (q == 0.0f ? 0.0f : 1.0f / q) : -q;
+    }
+  }
+  for (int c = 0; c < ncomp; ++c) {
+    float4 v[kPairs];
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j) v[j] = src[(size_t)c * M2 + at[j]];
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j) {
+      const unsigned idx = base + j * kOpNT;
+      if (idx < M2) {
+        float4 o;
+        if (INV) {
+          o.x = -v[j].x * f[j][0];
+          o.y = -v[j].y * f[j][0];
+          o.z = -v[j].z * f[j][1];
+          o.w = -v[j].w * f[j][1];
+        } else {
+          o.x = v[j].x * f[j][0];
+          o.y = v[j].y * f[j][0];
+          o.z = v[j].z * f[j][1];
+          o.w = v[j].w * f[j][1];
+        }
+        dst[(size_t)c * M2 + idx] = o;
+      }
+    }
+  }
+}
+
 // ---- Helmholtz (Leray) projection ----------------------------------------------
 // L = (a.u) / (a.a == 0 ? 1 : a.a);  u_i <- alpha g (u_i - a_i L) (+ beta d_i)
 //   (_helmholtz_projection_kernel!, :191-230)
@@ -417,6 +469,15 @@ gbg_status op_laplacian(gbg_plan* p, void* dst, const void* src, int ncomp, int 
     else
       hipLaunchKernelGGL((k_laplacian<double, false>), nblocks_span<double>(M), kOpNT, 0, s, (cpx<double>*)dst,
                          (const cpx<double>*)src, M, ncomp, modes_of(p), tables_of(p));
+  } else if (ncomp == 1 && M % 2 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
+    const unsigned M2 = M / 2;
+    const unsigned nb = (unsigned)((M2 + 2 * kOpNT - 1) / (2 * kOpNT));
+    if (inverse)
+      hipLaunchKernelGGL((k_laplacian_pair<true>), nb, kOpNT, 0, s, (float4*)dst, (const float4*)src,
+                         M2, ncomp, modes_of(p), tables_of(p));
+    else
+      hipLaunchKernelGGL((k_laplacian_pair<false>), nb, kOpNT, 0, s, (float4*)dst, (const float4*)src,
+                         M2, ncomp, modes_of(p), tables_of(p));
   } else {
     if (inverse)
       hipLaunchKernelGGL((k_laplacian<float, true>), nblocks_span<float>(M), kOpNT, 0, s, (cpx<float>*)dst,
