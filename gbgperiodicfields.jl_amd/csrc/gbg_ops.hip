@@ -80,6 +80,33 @@ __global__ __launch_bounds__(kOpNT) void k_gradient(cpx<T>* __restrict__ dst,
   }
 }
 
+// FP32 scalar -> vector with an even mode count and 16-byte aligned buffers:
+// two consecutive modes per lane, one 16-byte load and three 16-byte stores,
+// the same per-mode arithmetic as k_gradient.
+__global__ __launch_bounds__(kOpNT) void k_gradient_pair(float4* __restrict__ dst,
+                                                         const float4* __restrict__ src,
+                                                         unsigned M2, ModeIdx mi, Tables t) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= M2) return;
+  float a[2][3];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    unsigned kx, ky, kz;
+    mi(2 * idx + h, kx, ky, kz);
+    a[h][0] = float(t.kd1[kx]);
+    a[h][1] = float(t.kd2[ky]);
+    a[h][2] = float(t.kd3[kz]);
+  }
+  const float4 v = src[idx];
+  const cpx<float> v0 = mk<float>(v.x, v.y), v1 = mk<float>(v.z, v.w);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const cpx<float> c0 = cmul(mk<float>(0.0f, a[0][j]), v0);
+    const cpx<float> c1 = cmul(mk<float>(0.0f, a[1][j]), v1);
+    dst[(size_t)j * M2 + idx] = make_float4(c0.x, c0.y, c1.x, c1.y);
+  }
+}
+
 // ---- divergence --------------------------------------------------------------
 // i ((a1 s1 + a2 s2) + a3 s3), optionally alpha * g * (.) + beta * dst
 //   (_apply_spectral_divergence!, :120-139)
@@ -417,6 +444,9 @@ gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, h
   if (p->elem == 8)
     hipLaunchKernelGGL((k_gradient<double>), nblocks(M), kOpNT, 0, s, (cpx<double>*)dst,
                        (const cpx<double>*)src, M, nsrc, modes_of(p), tables_of(p));
+  else if (nsrc == 1 && M % 2 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0)
+    hipLaunchKernelGGL(k_gradient_pair, nblocks(M / 2), kOpNT, 0, s, (float4*)dst,
+                       (const float4*)src, M / 2, modes_of(p), tables_of(p));
   else
     hipLaunchKernelGGL((k_gradient<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
                        (const cpx<float>*)src, M, nsrc, modes_of(p), tables_of(p));
