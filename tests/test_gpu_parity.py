@@ -239,6 +239,40 @@ def test_uxw_parity(G, n, kc, dtype, fused):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,L,kc", [((64, 32, 128), (1.0, 2.5, 7.0), 20.0),
+                                    ((32, 64, 64), (3.0, 0.5, 2.0), 40.0),
+                                    ((64, 64, 64), (O.TWO_PI,) * 3, 0.0),
+                                    ((32, 32, 32), (O.TWO_PI,) * 3, 0.5)])
+def test_uxw_boxes_and_tiny_cutoffs(G, n, L, kc, dtype):
+    """Fused u x omega on boxes with unequal, non-2 pi sides (the mask then
+    compares non-integer |k|^2 in FP64) and with cutoffs that keep only the
+    mean mode or nothing beyond it: against the oracle, the mask bit-exact,
+    the reductions within tolerance (exactly zero when nothing is kept)."""
+    say(f"[uxw-boxes] n={n} L={L} kc={kc} dtype={np.dtype(dtype).name}")
+    g = ogrid(n, L)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n, L)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = O.synthetic_field(g, 2005, vector=True, m=m)
+    ref, E, Z = O.uxw(u, g, kc, m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    dn = G.Field(grid, "spectral", 1, dtype)
+    e, z = G.uxw_(dn, du, plans, kc, fused=True)
+    got = dn.download()
+    tol = TOL[dtype]
+    mask = O.band_mask(m, 0.0, kc)
+    assert np.all(got[:, mask == 0] == 0)
+    if np.linalg.norm(ref) > 0:
+        assert O.rel_l2(got, ref) < tol
+    else:
+        assert np.all(got == 0)
+    if E > 0:
+        assert abs(e - E) <= tol * abs(E) and abs(z - Z) <= tol * abs(Z)
+    else:
+        assert e == 0.0 and z == 0.0
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [(32, 32, 32), (64, 64, 256), (64, 64, 512), (32, 32, 1024)])
 def test_uxw_wide_offsets(G, n, dtype, monkeypatch):
     """The z-inverse pass of u x omega addresses a component with 32-bit element
