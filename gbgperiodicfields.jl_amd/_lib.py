@@ -73,6 +73,7 @@ SIGNATURES = {
     "gbg_reality_residual": (st, [vp, vp, i32, dp, vp]),
     "gbg_reality_projection": (st, [vp, vp, vp, i32, vp]),
     "gbg_uxw": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
+    "gbg_uxw_noalloc": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
     "gbg_dealiased_product": (st, [vp, vp, vp, vp, dbl, vp]),
     "gbg_plan_autotune": (st, [vp, vp, dp]),
     "gbg_batch_create": (st, [i32, i32, C.POINTER(vp)]),

@@ -266,6 +266,33 @@ static bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
   return x < y + nb && y < x + na;
 }
 
+namespace gbg {
+
+gbg_status uxw_check_args(gbg_plan* p, const void* u_hat, const void* n_hat, double kc) {
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!u_hat || !n_hat) return fail(GBG_E_ARG, "null field buffer");
+  if (!std::isfinite(kc) || kc < 0) return fail(GBG_E_ARG, "cutoff must be finite and >= 0");
+  const size_t b = 3 * p->M * 2 * p->elem;
+  if (overlaps(u_hat, b, n_hat, b)) return fail(GBG_E_ALIAS, "u_hat and n_hat must not overlap");
+  return GBG_OK;
+}
+
+gbg_status uxw_reductions(gbg_plan* p, double* energy, double* enstrophy, hipStream_t s) {
+  if (!energy && !enstrophy) return GBG_OK;
+  double h[2] = {0, 0};
+  gbg_status st =
+      check_hip(hipMemcpyAsync(h, p->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+                "reductions d2h");
+  if (st) return st;
+  if ((st = check_hip(hipStreamSynchronize(s), "reductions sync"))) return st;
+  if (energy) *energy = 0.5 * h[0];
+  if (enstrophy) *enstrophy = 0.5 * h[1];
+  return GBG_OK;
+}
+
+}  // namespace gbg
+
 extern "C" {
 
 const char* gbg_last_error(void) { return g_err.c_str(); }
@@ -697,23 +724,11 @@ gbg_status gbg_reality_projection(gbg_plan* p, void* dst, const void* src, int n
 gbg_status gbg_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, double* energy,
                    double* enstrophy, unsigned flags, void* stream) {
   GBG_GUARD_BEGIN
-  gbg_status st = need_plan(p);
+  gbg_status st = uxw_check_args(p, u_hat, n_hat, kc);
   if (st) return st;
-  if (!u_hat || !n_hat) return fail(GBG_E_ARG, "null field buffer");
-  if (!std::isfinite(kc) || kc < 0) return fail(GBG_E_ARG, "cutoff must be finite and >= 0");
-  const size_t b = 3 * p->M * 2 * p->elem;
-  if (overlaps(u_hat, b, n_hat, b)) return fail(GBG_E_ALIAS, "u_hat and n_hat must not overlap");
   hipStream_t s = as_stream(stream);
   if ((st = pipe_uxw(p, u_hat, n_hat, kc, flags, s))) return st;
-  if (energy || enstrophy) {
-    double h[2] = {0, 0};
-    if ((st = check_hip(hipMemcpyAsync(h, p->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "reductions d2h")))
-      return st;
-    if ((st = check_hip(hipStreamSynchronize(s), "reductions sync"))) return st;
-    if (energy) *energy = 0.5 * h[0];
-    if (enstrophy) *enstrophy = 0.5 * h[1];
-  }
-  return GBG_OK;
+  return uxw_reductions(p, energy, enstrophy, s);
   GBG_GUARD_END
 }
 

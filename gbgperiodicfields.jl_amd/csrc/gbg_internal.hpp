@@ -202,10 +202,20 @@ gbg_status op_reality_residual(gbg_plan* p, const void* spec, int ncomp, double*
                                hipStream_t s);
 
 // pipelines (gbg_uxw.hip)
+// the argument checks of gbg_uxw (null buffers, cutoff, u_hat / n_hat overlap)
+gbg_status uxw_check_args(gbg_plan* p, const void* u_hat, const void* n_hat, double kc);
+// energy and enstrophy of the last u x omega call on p into the host doubles
+// (either may be NULL; when both are, nothing happens): synchronises s
+gbg_status uxw_reductions(gbg_plan* p, double* energy, double* enstrophy, hipStream_t s);
 gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
                     hipStream_t s);
 gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
                                   double kc, hipStream_t s);
+// pipe_uxw with the caller's work area of work_bytes (refused with
+// GBG_E_UNSUPPORTED when smaller than uxw_work_bytes): creates nothing on
+// the device (no work area, tile set or stream)
+gbg_status pipe_uxw_in(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
+                       void* work, size_t work_bytes, hipStream_t s);
 // work-area bytes pipe_uxw needs on this plan's grid for cutoff kc (reads
 // only the plan's n, L, nh, M, P, elem and generic_only)
 size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags);

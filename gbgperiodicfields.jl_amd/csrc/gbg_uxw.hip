@@ -36,14 +36,13 @@ namespace gbg {
 // ============================================================================
 // composed path
 // ============================================================================
-gbg_status pipe_uxw_composed(gbg_plan* p, const void* u_hat, void* n_hat, double kc,
-                             hipStream_t s) {
+// w: 2 (3 M s_c) + 2 (3 P s) bytes of work area
+static gbg_status uxw_composed(gbg_plan* p, const void* u_hat, void* n_hat, double kc, char* w,
+                               hipStream_t s) {
   const size_t cs = (size_t)p->elem * 2;  // complex element bytes
   const size_t rs = (size_t)p->elem;      // real element bytes
   const size_t spec3 = 3 * p->M * cs, phys3 = 3 * p->P * rs;
-  gbg_status st = ensure_work(p, 2 * spec3 + 2 * phys3);
-  if (st) return st;
-  char* w = (char*)p->work;
+  gbg_status st = GBG_OK;
   void* ut = w;
   void* wt = w + spec3;
   void* u = w + 2 * spec3;
@@ -1036,14 +1035,16 @@ static Geo make_geo(gbg_plan* p, double kc) {
   return g;
 }
 
-// The plan's tile set for cutoff g.hi2 and tile width b, built on first use.
-// Returns nullptr (the last pass then classifies tiles itself) when the set
-// is not cached and cannot be built now: the stream is being captured, or
-// the plan already holds kTileSets sets.
-static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipStream_t s) {
+// The plan's tile set for cutoff g.hi2 and tile width b, built on first use
+// when `build` holds.  Returns nullptr (the last pass then classifies tiles
+// itself) when the set is not cached and is not built now: `build` is false
+// (a caller that must not allocate), the stream is being captured, or the
+// plan already holds kTileSets sets.
+static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipStream_t s,
+                               bool build = true) {
   for (int i = 0; i < p->ntset; ++i)
     if (p->tset[i].kc == kc && p->tset[i].b == b) return &p->tset[i];
-  if (p->ntset >= kTileSets) return nullptr;
+  if (!build || p->ntset >= kTileSets) return nullptr;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
   std::vector<double> s1, s2;
@@ -1186,18 +1187,18 @@ static gbg_status zero_tiles_join(gbg_plan* p, hipStream_t s) {
   return check_hip(hipStreamWaitEvent(s, p->aux_join, 0), "join of the zero-tile stream");
 }
 
+// base: uxw_work_bytes of work area.  With noalloc the pass schedule creates
+// nothing (no tile set, no second stream): the last pass uses a tile set only
+// when the plan already caches one for this cutoff.
 template <typename T>
 static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
-                            double kc, hipStream_t s) {
+                            double kc, char* base, bool noalloc, hipStream_t s) {
   const size_t cb = sizeof(cpx<T>);
   // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
   // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields
   const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cb;
   const size_t b2 = (size_t)6 * g.s2 * cb;
-  const long long part_max = (long long)g.nyc * g.nhp + fast::kXcd;   // >= K1 grid
-  gbg_status st = ensure_work(p, b1 + b2 + sizeof(double) * 2 * part_max);
-  if (st) return st;
-  char* base = (char*)p->work;
+  gbg_status st = GBG_OK;
   cpx<T>* W1 = (cpx<T>*)base;
   cpx<T>* W2 = (cpx<T>*)(base + b1);
   double* part = (double*)(base + b1 + b2);
@@ -1207,7 +1208,8 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   const cpx<T>* twz = (const cpx<T>*)p->tw[2];
   const double P = (double)g.nx * g.ny * g.nz;
   const TileSet* fork =
-      zero_tiles_fork<T>(p, g, kc, 3, out, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
+      noalloc ? nullptr
+              : zero_tiles_fork<T>(p, g, kc, 3, out, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
   prof_begin(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
@@ -1266,7 +1268,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     constexpr int N = decltype(L)::value;
     using C = Cfg<T, N>;
     const long long plane = (long long)g.ny * g.nh;
-    const TileSet* ts = tile_set(p, g, C::B, kc, s);
+    const TileSet* ts = tile_set(p, g, C::B, kc, s, !noalloc);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
       launch_z_forward<T, N, 3>(p, g, nt, W2, out, tb, twz, nullptr, s);
@@ -1282,26 +1284,48 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   return GBG_OK;
 }
 
+static bool uxw_fusable(const gbg_plan* p, unsigned flags) {
+  return !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
+         fast_len(p->n[1]) && fast_len(p->n[2]);
+}
+
 size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags) {
-  const bool fusable = !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
-                       fast_len(p->n[1]) && fast_len(p->n[2]);
   const size_t cs = (size_t)p->elem * 2, rs = (size_t)p->elem;
-  if (!fusable) return 2 * (3 * p->M * cs) + 2 * (3 * p->P * rs);
+  if (!uxw_fusable(p, flags)) return 2 * (3 * p->M * cs) + 2 * (3 * p->P * rs);
   const Geo g = make_geo(p, kc);
+  // W1, W2 and the K1 partials (2 doubles per launched workgroup, at most
+  // one per tile plus the XCD padding)
   const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cs;
   const size_t b2 = (size_t)6 * g.s2 * cs;
   return b1 + b2 + sizeof(double) * 2 * ((size_t)g.nyc * g.nhp + fast::kXcd);
 }
 
-gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
-                    hipStream_t s) {
-  const bool fusable = !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
-                       fast_len(p->n[1]) && fast_len(p->n[2]);
-  if (!fusable) return pipe_uxw_composed(p, u_hat, n_hat, kc, s);
+static gbg_status uxw_run(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
+                          char* work, bool noalloc, hipStream_t s) {
+  if (!uxw_fusable(p, flags)) return uxw_composed(p, u_hat, n_hat, kc, work, s);
   const Geo g = make_geo(p, kc);
   if (p->elem == 8)
-    return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, kc, s);
-  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, s);
+    return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, kc, work,
+                             noalloc, s);
+  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, work, noalloc,
+                          s);
+}
+
+gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
+                    hipStream_t s) {
+  gbg_status st = ensure_work(p, uxw_work_bytes(p, kc, flags));
+  if (st) return st;
+  return uxw_run(p, u_hat, n_hat, kc, flags, (char*)p->work, false, s);
+}
+
+gbg_status pipe_uxw_in(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
+                       void* work, size_t work_bytes, hipStream_t s) {
+  const size_t need = uxw_work_bytes(p, kc, flags);
+  if (need > work_bytes)
+    return fail(GBG_E_UNSUPPORTED, "u x omega on this grid and cutoff needs " + std::to_string(need) +
+                                       " bytes of work area; the caller's area holds " +
+                                       std::to_string(work_bytes));
+  return uxw_run(p, u_hat, n_hat, kc, flags, (char*)work, true, s);
 }
 
 // Fused scalar product p^ = M F(F^-1(M a^) F^-1(M b^)), the five passes of

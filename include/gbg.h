@@ -188,6 +188,18 @@ gbg_status gbg_reality_projection(gbg_plan* plan, void* dst, const void* src, in
 #define GBG_UXW_UNFUSED 1u
 gbg_status gbg_uxw(gbg_plan* plan, const void* u_hat, void* n_hat, double kc, double* energy,
                    double* enstrophy, unsigned flags, void* stream);
+/* gbg_uxw for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, kernels and results as
+ * gbg_uxw, but its work area is a fixed 128 MiB device array of the library
+ * (one per device, held until the process ends) and the call creates nothing
+ * on the device: no hipMalloc, no tile list, no stream.  Grids and cutoffs
+ * whose work area exceeds 128 MiB are refused with GBG_E_UNSUPPORTED (it
+ * holds 128^3 FP64 at the 2/3 cutoff, 128^3 FP32 at any cutoff).  The plan
+ * must belong to the current device.  Every call on a device shares the
+ * array: calls that overlap in time on one device race. */
+gbg_status gbg_uxw_noalloc(gbg_plan* plan, const void* u_hat, void* n_hat, double kc,
+                           double* energy, double* enstrophy, unsigned flags, void* stream);
 /* Dealiased scalar product p^ = M F(F^-1(M a^) * F^-1(M b^)).  Inputs read only. */
 gbg_status gbg_dealiased_product(gbg_plan* plan, void* p_hat, const void* a_hat, const void* b_hat,
                                  double kc, void* stream);

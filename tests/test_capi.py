@@ -64,6 +64,7 @@ def test_null_and_value_refusals_without_gpu(gbg):
     assert b"positive" in L.gbg_last_error()
     assert L.gbg_to_spectral(None, None, None, 1, None) == 1     # null plan
     assert L.gbg_uxw(None, None, None, 1.0, None, None, 0, None) == 1
+    assert L.gbg_uxw_noalloc(None, None, None, 1.0, None, None, 0, None) == 1
     assert L.gbg_free(None) == 0
 
 
