@@ -267,6 +267,32 @@ def band_mask(m: Multipliers, k_lo: float, k_hi: float) -> np.ndarray:
     return ((lo2 <= ks) & (ks <= hi2)).astype(np.float64)
 
 
+def band_mask_float32(grid: Grid3D, k_lo: float, k_hi: float) -> np.ndarray:
+    """The band mask as the reference's Float32 path evaluates it: per-axis
+    tables in Float32 (``_axis_wavenumber_values``, ``src/Fields.jl:2006-2026``
+    with T = Float32: ``unit = 2*T(pi)/L``, ``k = unit*n``, ``k^2``), ``|k|^2 =
+    (k1^2 + k2^2) + k3^2`` in Float32 (``src/Fields.jl:2105-2109``) and edges
+    ``T(lo)^2``, ``T(hi)^2`` (``src/Convolution.jl:1832-1850``).  The library's
+    FP32 form keeps the FP64 predicate of ``band_mask`` (mode handling in
+    FP64); this restatement shows where the two select different boundary
+    modes.  Returns a bool array over (Nz, Ny, Nh)."""
+    T = np.float32
+
+    def sq(n, L, half):
+        unit = T(2) * T(math.pi) / T(L)
+        stored = n // 2 + 1 if half else n
+        i = np.arange(stored)
+        freq = i if half else np.where(i < (n + 1) // 2, i, i - n)
+        k = unit * freq.astype(T)
+        return k * k
+
+    s1 = sq(grid.n[0], grid.L[0], True)
+    s2 = sq(grid.n[1], grid.L[1], False)
+    s3 = sq(grid.n[2], grid.L[2], False)
+    q = (s1[None, None, :] + s2[None, :, None]) + s3[:, None, None]
+    return (T(k_lo) ** 2 <= q) & (q <= T(k_hi) ** 2)
+
+
 def mask_digest(m: Multipliers, kc: float) -> tuple[int, str]:
     """Kept-mode count and SHA-256 of the bit-packed 2/3 mask of
     ``band_mask(m, 0, kc)`` in storage order (kx fastest), computed one kz

@@ -208,7 +208,9 @@ def test_operators(G, n, L, dtype):
 UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), ((48, 32, 40), None),
              ((64, 64, 128), None), ((128, 128, 32), None), ((64, 64, 64), 5.0), ((64, 64, 64), 1000.0),
              ((32, 32, 64), 7.3), ((512, 32, 32), 170.5), ((256, 64, 32), 60.0),
-             ((1024, 32, 32), 340.0), ((32, 1024, 32), 12.0), ((32, 32, 1024), 12.0)]
+             ((1024, 32, 32), 340.0), ((32, 1024, 32), 12.0), ((32, 32, 1024), 12.0),
+             # Ny = 512: the y passes of the 512^3 headline, k_uxw_y<T, 512, +-1, ...>
+             ((64, 512, 64), None), ((32, 512, 64), 150.0)]
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -270,6 +272,40 @@ def test_uxw_boxes_and_tiny_cutoffs(G, n, L, kc, dtype):
         assert abs(e - E) <= tol * abs(E) and abs(z - Z) <= tol * abs(Z)
     else:
         assert e == 0.0 and z == 0.0
+
+
+def _fp32_mask_case():
+    n, L = (64, 64, 64), (1.3, 1.3, 1.3)
+    g = ogrid(n, L)
+    m = O.Multipliers.build(g)
+    kc = 15 * (2 * math.pi / 1.3)
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((3,) + g.spectral_shape) + 1j * rng.standard_normal((3,) + g.spectral_shape)
+    ref, E, Z = O.uxw(u, g, kc, m)
+    return n, L, g, m, kc, u, ref, E, Z
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_uxw_fp32_mask_semantics(G, fused):
+    """FP32 u x omega on a 1.3 box with the cutoff on a lattice sphere (15
+    lattice units): the output keeps exactly the FP64 predicate's modes, 81
+    boundary modes more than the reference's Float32 predicate would (mode
+    handling stays in FP64 in the FP32 form)."""
+    say(f"[uxw-fp32-mask] 64^3 L=1.3 kc=15 units fused={fused}")
+    n, L, g, m, kc, u, ref, E, Z = _fp32_mask_case()
+    grid = G.Grid3D(n, L)
+    plans = G.FieldTransformPlans(grid, np.float32)
+    du = G.Field(grid, "spectral", 1, np.float32).upload(u)
+    dn = G.Field(grid, "spectral", 1, np.float32)
+    e, z = G.uxw_(dn, du, plans, kc, fused=fused)
+    got = dn.download()
+    assert O.rel_l2(got, ref) < 1e-5
+    kept = np.any(got != 0, axis=0)
+    kept[0, 0, 0] = True
+    m64 = O.band_mask(m, 0.0, kc).astype(bool)
+    m32 = O.band_mask_float32(g, 0.0, kc)
+    assert np.array_equal(kept, m64)
+    assert int((kept & ~m32).sum()) == 81
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

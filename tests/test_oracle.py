@@ -173,9 +173,14 @@ def test_band_membership_and_errors():
 
 
 def test_mask_golden_64():
-    """Kept-mode count at 64^3 and 512^3 (closed form count of integer
-    lattice points with |n|^2 <= floor(kc^2), n_x >= 0)."""
-    for N in (64, 512):
+    """Kept-mode count at 64^3, 512^3 and 1024^3 (closed form count of integer
+    lattice points with |n|^2 <= floor(kc^2), n_x >= 0): equal to the oracle's
+    mask at 64^3 and to the committed digests' kept counts at every size."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "mask_digests.json")) as fh:
+        digests = json.load(fh)
+    for N in (64, 512, 1024):
         kc2 = math.floor(O.dealias_cutoff(N) ** 2)
         r = math.isqrt(kc2)
         cnt = 0
@@ -184,6 +189,7 @@ def test_mask_golden_64():
                 rem = kc2 - nx * nx - ny * ny
                 if rem >= 0:
                     cnt += 2 * math.isqrt(rem) + 1
+        assert digests[f"{N}x{N}x{N}"]["kept"] == cnt, N
         if N == 64:
             g = O.Grid3D((N, N, N))
             assert int(O.band_mask(O.Multipliers.build(g), 0.0, O.dealias_cutoff(N)).sum()) == cnt
@@ -273,3 +279,51 @@ def test_synthetic_field_properties():
     # seeded: deterministic and seed-sensitive
     assert np.array_equal(O.synthetic_field(g, 2001, True, m), u)
     assert not np.array_equal(O.synthetic_field(g, 2002, True, m), u)
+
+
+# --- the reference's Float32 mask predicate vs the FP64 one ---------------------
+
+def test_float32_mask_restatement_by_scalar_loop():
+    """band_mask_float32 against a per-mode loop in float32 scalars on a small
+    non-2 pi box with a cutoff on a lattice sphere."""
+    n, L = (8, 6, 10), (1.3, 0.9, 2.0)
+    g = O.Grid3D(n, L)
+    kc = 3.0 * (2 * math.pi / 1.3)
+    got = O.band_mask_float32(g, 0.0, kc)
+    f = np.float32
+    unit = [f(2) * f(math.pi) / f(v) for v in L]
+    hi = f(kc) * f(kc)
+    for kz in range(n[2]):
+        for ky in range(n[1]):
+            for kx in range(n[0] // 2 + 1):
+                fy = ky if ky < (n[1] + 1) // 2 else ky - n[1]
+                fz = kz if kz < (n[2] + 1) // 2 else kz - n[2]
+                a, b, c = unit[0] * f(kx), unit[1] * f(fy), unit[2] * f(fz)
+                q = f(f(a * a) + f(b * b)) + f(c * c)
+                assert got[kz, ky, kx] == (f(0) <= q <= hi)
+
+
+@pytest.mark.parametrize("n", [(64, 64, 64), (32, 32, 32), (48, 48, 48), (96, 96, 96),
+                               (128, 128, 256), (256, 64, 64)])
+def test_float32_mask_equals_fp64_on_2pi_boxes(n):
+    """On the 2 pi box every |k|^2 is an integer, exact in Float32 and FP64
+    alike, so the reference's Float32 mask and the FP64 mask the library
+    applies in both precisions select the same modes (the workload grids)."""
+    g = O.Grid3D(n)
+    kc = O.dealias_cutoff(min(n))
+    a = O.band_mask(O.Multipliers.build(g), 0.0, kc).astype(bool)
+    assert np.array_equal(a, O.band_mask_float32(g, 0.0, kc))
+
+
+def test_float32_mask_differs_on_lattice_spheres():
+    """Off the 2 pi box, a cutoff on a lattice sphere (here 15 lattice units
+    on a 1.3 box) puts modes within Float32 rounding of the edge: the
+    reference's Float32 predicate drops 81 boundary modes the FP64 predicate
+    keeps.  The library keeps the FP64 predicate in FP32 as well (the GPU
+    test of the same case checks that)."""
+    g = O.Grid3D((64, 64, 64), (1.3, 1.3, 1.3))
+    kc = 15 * (2 * math.pi / 1.3)
+    a = O.band_mask(O.Multipliers.build(g), 0.0, kc).astype(bool)
+    b = O.band_mask_float32(g, 0.0, kc)
+    assert int(a.sum()) == 7428 and int((a & ~b).sum()) == 81 and not np.any(b & ~a)
+

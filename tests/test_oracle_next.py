@@ -8,6 +8,7 @@ import math
 import numpy as np
 import pytest
 
+from closed_forms import abc_modes, dense, shear_modes
 from oracle import gbg_oracle as O
 
 TWO_PI = 2 * math.pi
@@ -223,3 +224,27 @@ def test_reality_round_trip_is_the_projection():
     rt = O.to_spectral(O.to_physical(f, g))
     np.testing.assert_allclose(rt, O.reality_projection(f, g), atol=1e-13)
     assert O.reality_residual(O.to_spectral(rng.standard_normal(g.physical_shape)), g) < 1e-12
+
+
+def test_closed_forms_pinned_by_oracle():
+    """The mode lists of the 1024^3 closed-form GPU tests (tests/closed_forms.py)
+    are what the oracle gives on 8^3: the input spectra equal the transform of
+    the physical fields, the shear pair's projected product and the ABC
+    flow's vanishing product come out of the oracle's u x omega, and so do
+    the closed-form energy and enstrophy."""
+    n = (8, 8, 8)
+    g = O.Grid3D(n)
+    x = np.arange(8) * (O.TWO_PI / 8)
+    Z3, Y3, X3 = np.meshgrid(x, x, x, indexing="ij")
+    u = np.stack([np.sin(Z3), np.sin(X3), 0 * X3])
+    us, ns = shear_modes(512.0)
+    assert np.abs(O.to_spectral(u) - dense(us, n)).max() < 1e-12
+    got, E, Z = O.uxw(dense(us, n), g, O.dealias_cutoff(1024))
+    assert np.abs(got - dense(ns, n)).max() < 1e-12 and abs(E - 0.5) < 1e-15 and abs(Z - 0.5) < 1e-15
+    A, B, Cc = 1.0, 0.7, 0.4
+    abc = np.stack([A * np.sin(Z3) + Cc * np.cos(Y3), B * np.sin(X3) + A * np.cos(Z3),
+                    Cc * np.sin(Y3) + B * np.cos(X3)])
+    assert np.abs(O.to_spectral(abc) - dense(abc_modes(512.0, A, B, Cc), n)).max() < 1e-12
+    got, E, Z = O.uxw(dense(abc_modes(512.0, A, B, Cc), n), g, O.dealias_cutoff(1024))
+    assert np.abs(got).max() < 1e-12
+    assert abs(E - (A * A + B * B + Cc * Cc) / 2) < 1e-14 and abs(Z - E) < 1e-14
