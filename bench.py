@@ -66,21 +66,67 @@ class HipEvents:
         return float(ms.value)
 
 
-def uxw_bytes(n, elem):
-    """Algorithmic bytes of one u x omega term: 42 component-sized transfers of
-    M = (Nx/2+1) Ny Nz complex elements (the fused five-pass ideal)."""
+def kept_ranges(n, kc, L=None):
+    """Index ranges the fused schedule keeps for cutoff kc, from FP64
+    squared-wavenumber tables and the predicate (k1^2 + k2^2) + k3^2 <= kc^2
+    the library applies: nhk kept kx (x folded), nyc kept ky, disk = kept
+    (kx, ky) pencils, ball = kept modes."""
+    L = L or (2.0 * math.pi,) * 3
+
+    def sq(m, length, half):
+        unit = (2.0 * math.pi) / length
+        i = np.arange(m // 2 + 1 if half else m)
+        f = i if half else np.where(i < (m + 1) // 2, i, i - m)
+        k = unit * f.astype(np.float64)
+        return k * k
+
+    s1, s2, s3 = sq(n[0], L[0], True), sq(n[1], L[1], False), sq(n[2], L[2], False)
+    hi2 = float(kc) * float(kc)
+    nhk = max(1, int(np.count_nonzero(s1 <= hi2)))      # s1 grows with kx
+    plane = s1[None, :] + s2[:, None]                 # (Ny, Nh)
+    disk = int(np.count_nonzero(plane[:, :nhk] <= hi2))
+    ball = sum(int(np.count_nonzero(plane + z <= hi2)) for z in s3)
+    return {"nhk": nhk, "nyc": int(np.count_nonzero(s2 <= hi2)), "disk": disk, "ball": ball}
+
+
+def uxw_stage_bytes(n, kc, elem):
+    """Algorithmic bytes of each pass of the fused u x omega schedule: what the
+    pass must read and write given the index ranges the 2/3 mask keeps
+    (complex elements of 2 elem bytes).  With row = Nz Ny nhk (full-y rows of
+    kept kx) and disk = Nz x (kept (kx, ky) pencils):
+      K1  read u^ on the kept ball (3 comps), write u, w on the disk (6)
+      K2  read 6 on the disk, write 6 full-y rows
+      K3  read 6 rows, write 3 rows (u x w transformed back)
+      K4  read 3 rows, write 3 on the disk
+      K5  read 3 on the disk, write the whole output (3 M, zeros included)."""
+    r = kept_ranges(n, kc)
+    sc = 2 * elem
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    row = n[2] * n[1] * r["nhk"]
+    disk = n[2] * r["disk"]
+    return {"K1_z_inverse_curl": (3 * r["ball"] + 6 * disk) * sc,
+            "K2_y_inverse": (6 * disk + 6 * row) * sc,
+            "K3_x_product": 9 * row * sc,
+            "K4_y_forward": (3 * row + 3 * disk) * sc,
+            "K5_z_forward_project": (3 * disk + 3 * M) * sc}
+
+
+def product_bytes(n, kc, elem):
+    """Algorithmic bytes of the fused dealiased scalar product, as above with
+    two fields in and one out: P1 2 ball + 2 disk, P2 2 disk + 2 rows,
+    P3 3 rows, P4 rows + disk, P5 disk + M."""
+    r = kept_ranges(n, kc)
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    row = n[2] * n[1] * r["nhk"]
+    disk = n[2] * r["disk"]
+    return (2 * r["ball"] + 2 * disk + 2 * disk + 2 * row + 3 * row + row + disk + disk + M) * 2 * elem
+
+
+def uxw_bytes_unpruned(n, elem):
+    """42 component-sized transfers of M = (Nx/2+1) Ny Nz complex elements: the
+    five passes as if nothing were masked (an equivalent-rate label only)."""
     M = (n[0] // 2 + 1) * n[1] * n[2]
     return 42 * M * 2 * elem
-
-
-def stage_bytes(n, elem):
-    """Algorithmic bytes of each pass of the fused schedule, in component
-    transfers (reads + writes) of M complex elements."""
-    M = (n[0] // 2 + 1) * n[1] * n[2]
-    comp = M * 2 * elem
-    return {"K1_z_inverse_curl": 9 * comp, "K2_y_inverse": 12 * comp,
-            "K3_x_product": 9 * comp, "K4_y_forward": 6 * comp,
-            "K5_z_forward_project": 6 * comp}
 
 
 KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}, true>",
@@ -133,13 +179,15 @@ def run_gpu(G, n, dtype, steps, warmup, stream, ev, seed=2001):
     wall = time.perf_counter() - t0
     per = [ev.elapsed(evs[i], evs[i + 1]) for i in range(steps)]
     total_ms = ev.elapsed(evs[0], evs[-1])
-    # per-pass durations, measured with events on the same stream
+    # per-pass durations: HIP events the library records between its kernels
+    # on the same stream, averaged over 16 terms
     plans.set_profiling(True)
-    for _ in range(min(steps, 16)):
+    runs = []
+    for _ in range(16):
         G.uxw_(dn, du, plans, kc, reductions=False, stream=stream)
-    stream.synchronize()
-    stages = plans.stage_times()
+        runs.append(plans.stage_times())
     plans.set_profiling(False)
+    stages = [float(np.mean(col)) for col in zip(*runs)]
     E, Z = G.uxw_(dn, du, plans, kc, reductions=True, stream=stream)
     return dict(plans=plans, du=du, dn=dn, per=per, total_ms=total_ms, wall=wall, stages=stages,
                 E=E, Z=Z, gen_s=gen_s, kc=kc)
@@ -172,8 +220,13 @@ def run_batch(G, stream, ev, steps, warmup, nitems=96, seed=4001):
             grid = G.Grid3D(n)
             d = G.Field(grid, "spectral", int(k), dtype).upload(f)
             (batch.add_vector if k else batch.add_scalar)(d, G.dealias_cutoff(min(n)))
+            # gradient (4 M or 12 M component transfers) plus the pruned
+            # u x omega or dealiased product passes
+            es = np.dtype(dtype).itemsize
             M = (n[0] // 2 + 1) * n[1] * n[2]
-            alg += (54 if k else 19) * M * 2 * np.dtype(dtype).itemsize
+            kcb = G.dealias_cutoff(min(n))
+            alg += (12 * M * 2 * es + sum(uxw_stage_bytes(n, kcb, es).values()) if k
+                    else 4 * M * 2 * es + product_bytes(n, kcb, es))
         t0 = time.time()
         batch.instantiate(stream)
         stream.synchronize()
@@ -317,7 +370,7 @@ def run_rows(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
         ("cross (physical)", 9 * ph, lambda: G.cross_(pw, pu, pv, plans, stream=stream)),
         ("multiply (physical scalar)", 3 * ph, lambda: G.multiply_(pc, pa, pb, plans, stream=stream)),
         ("mean_free_projection (vector)", None, lambda: G.mean_free_projection_(dv, plans, stream=stream)),
-        ("dealiased_product (scalar)", 15 * sc,
+        ("dealiased_product (scalar)", product_bytes(n, G.dealias_cutoff(n[0]), es),
          lambda: G.dealiased_product_(ds, da, db, plans, stream=stream)),
     ]
     res = {}
@@ -495,6 +548,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-reps", type=int, default=3,
+                    help="timed CPU-baseline terms per precision after one warm-up")
     ap.add_argument("--form", default="uxw",
                     choices=["uxw", "batch", "serve", "hbm", "ops", "rows"],
                     help="uxw: the headline 512^3 term (default); batch: the 96-field batched "
@@ -608,15 +663,14 @@ def main():
     r = run_gpu(G, n, dtype, args.steps, args.warmup, stream, ev)
     ms = r["total_ms"] / args.steps
     fields_s = 1000.0 / ms
-    B = uxw_bytes(n, elem)
     per = sorted(r["per"])
     p50 = per[len(per) // 2]
     p99 = per[min(len(per) - 1, int(math.ceil(0.99 * len(per))) - 1)]
     print(f"[bench] gpu {ms:.3f} ms/step, stages {r['stages']}", file=sys.stderr, flush=True)
 
-    # dominant pass and its roofline
-    names = list(stage_bytes(n, elem).keys())
-    sb = stage_bytes(n, elem)
+    # dominant pass and its roofline, on the bytes the pruned pass schedule moves
+    sb = uxw_stage_bytes(n, r["kc"], elem)
+    names = list(sb.keys())
     stages = r["stages"]
     roof = None
     if len(stages) == len(names):
@@ -624,10 +678,16 @@ def main():
         ach = sb[names[i]] / (stages[i] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": names[i], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(names[i], n, dtype), "avg_ms": round(stages[i], 4)}
+                "traffic": measured_traffic(names[i], n, dtype), "avg_ms": round(stages[i], 4),
+                "bytes_alg": int(sb[names[i]])}
+    B = sum(sb.values())
     pipeline_gbs = B / (ms * 1e-3) / 1e9
+    M = (n[0] // 2 + 1) * n[1] * n[2]
+    Bu = uxw_bytes_unpruned(n, elem)
+    prec = "FP64" if dtype == np.float64 else "FP32"
     out = {
-        "metric": "512^3 dealiased u x omega fields/s (FP64) + % HBM roofline; FP32 vs FP64; p99 latency",
+        "metric": f"{n[0]}^3 dealiased u x omega fields/s ({prec}) + % HBM roofline; FP32 vs FP64; "
+                  "p99 latency",
         "value": round(fields_s, 3),
         "unit": "fields/s",
         "n_gpus": 1,
@@ -643,27 +703,49 @@ def main():
                                "Leray projection, energy/enstrophy", "grid": list(n),
                    "kc": r["kc"], "parallelism": "single GPU"},
         "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4)},
-        "pipeline_roofline": {"bytes_alg": B, "achieved": round(pipeline_gbs, 1),
-                              "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
-        "stage_ms": [round(s, 4) for s in stages],
+        "pipeline_roofline": {"bytes_alg": int(B), "achieved": round(pipeline_gbs, 1),
+                              "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4),
+                              "stage_bytes_alg": {k: int(v) for k, v in sb.items()},
+                              "compulsory_bytes": int(6 * M * 2 * elem),
+                              "unpruned_equivalent": {
+                                  "bytes": int(Bu), "GBs": round(Bu / (ms * 1e-3) / 1e9, 1),
+                                  "note": "42 M s_c as if no mode were masked; a label, not a "
+                                          "roofline"}},
+        "stage_ms": [round(x, 4) for x in stages],
         "roofline": roof,
     }
     nh_gpu = r["dn"].download()
     E, Z = r["E"], r["Z"]
     del r
 
+    n32 = None
     if dtype == np.float64 and not args.no_fp32:
         r32 = run_gpu(G, n, np.float32, args.steps, args.warmup, stream, ev)
         ms32 = r32["total_ms"] / args.steps
         n32 = r32["dn"].download()
+        sb32 = uxw_stage_bytes(n, r32["kc"], 4)
         out["fp32"] = {"value": round(1000.0 / ms32, 3), "ms_per_step": round(ms32, 4),
                        "speedup_vs_fp64": round(ms / ms32, 3),
-                       "stage_ms": [round(s, 4) for s in r32["stages"]]}
+                       "stage_ms": [round(x, 4) for x in r32["stages"]],
+                       "pipeline_frac": round(sum(sb32.values()) / (ms32 * 1e-3) / 1e9
+                                              / HBM_PEAK_GBS, 4)}
+        E32, Z32 = r32["E"], r32["Z"]
         del r32
-    else:
-        n32 = None
 
-    if not args.no_cpu_baseline:
+    if args.no_cpu_baseline or n[0] > 512:
+        if n32 is not None:
+            # no host oracle at this size: FP32 against the FP64 GPU result,
+            # which the parity tests hold to 1e-12 of the FP64 oracle
+            num = den = 0.0
+            for c in range(3):
+                d = n32[c].astype(np.complex128) - nh_gpu[c]
+                num += float(np.vdot(d, d).real)
+                den += float(np.vdot(nh_gpu[c], nh_gpu[c]).real)
+                del d
+            out["rel_l2_fp32_vs_fp64_gpu"] = {"N_hat": math.sqrt(num / den),
+                                             "energy": abs(E32 - E) / abs(E),
+                                             "enstrophy": abs(Z32 - Z) / abs(Z)}
+    else:
         os.environ["GBG_ORACLE_WORKERS"] = str(args.cpu_workers)
         sys.path.insert(0, REPO)
         from oracle import gbg_oracle as O   # the checker and CPU baseline only
@@ -671,10 +753,23 @@ def main():
         g = O.Grid3D(n)
         m = O.Multipliers.build(g)
         u = workload.synthetic_field(n, 2001, vector=True)
-        t0 = time.perf_counter()
-        ref, Er, Zr = O.uxw(u, g, out["config"]["kc"], m, workers=args.cpu_workers)
-        cpu_s = time.perf_counter() - t0
+        kc = out["config"]["kc"]
+        reps = args.cpu_reps
+        t64 = []
+        for _ in range(reps + 1):                  # the first run is the warm-up
+            t0 = time.perf_counter()
+            ref, Er, Zr = O.uxw(u, g, kc, m, workers=args.cpu_workers)
+            t64.append(time.perf_counter() - t0)
+        u32 = u.astype(np.complex64)
         del u
+        t32 = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            ref32 = cpu_uxw_single(u32, m, kc, args.cpu_workers)
+            t32.append(time.perf_counter() - t0)
+        del u32
+        cpu_err32 = O.rel_l2(ref32.astype(np.complex128), ref)
+        del ref32
         cpu_model = platform.processor() or ""
         try:
             with open("/proc/cpuinfo") as fh:
@@ -684,19 +779,78 @@ def main():
                         break
         except OSError:
             pass
-        out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 4), "unit": "fields/s",
-                               "cores": args.cpu_workers, "kind": "port",
-                               "sample": f"one full {n[0]}^3 FP64 u x omega term (NumPy + scipy.fft "
-                                         f"pocketfft, workers={args.cpu_workers}), {cpu_s:.2f} s",
-                               "cpu_model": cpu_model}
+        w64, w32 = sorted(t64[1:]), sorted(t32[1:])
+        med64, med32 = float(np.median(w64)), float(np.median(w32))
+        out["cpu_baseline"] = {
+            "value": round(1.0 / med64, 4), "unit": "fields/s", "cores": args.cpu_workers,
+            "kind": "port",
+            "sample": f"{reps} timed full {n[0]}^3 u x omega terms after one warm-up, FP64 "
+                      f"(the oracle: NumPy + scipy.fft pocketfft, workers={args.cpu_workers}); "
+                      f"value from the median",
+            "fp64_s": {"min": round(w64[0], 3), "median": round(med64, 3), "runs": reps},
+            "fp32": {"value": round(1.0 / med32, 4), "min_s": round(w32[0], 3),
+                     "median_s": round(med32, 3), "runs": reps,
+                     "rel_l2_vs_fp64_oracle": cpu_err32},
+            "fft_library": "pocketfft (scipy.fft " + _scipy_version() + ")",
+            "fftw_on_host": _fftw_probe(),
+            "cpu_model": cpu_model, "host_cpus": os.cpu_count()}
         out["rel_l2_vs_oracle"] = {"N_hat_fp64": O.rel_l2(nh_gpu, ref),
                                    "energy": abs(E - Er) / abs(Er), "enstrophy": abs(Z - Zr) / abs(Zr)}
         if n32 is not None:
-            out["rel_l2_vs_oracle"]["N_hat_fp32"] = O.rel_l2(n32.astype(np.complex128), ref)
-        mask = O.band_mask(m, 0.0, out["config"]["kc"])
+            out["rel_l2_vs_oracle"].update({
+                "N_hat_fp32": O.rel_l2(n32.astype(np.complex128), ref),
+                "energy_fp32": abs(E32 - Er) / abs(Er), "enstrophy_fp32": abs(Z32 - Zr) / abs(Zr)})
+        mask = O.band_mask(m, 0.0, kc)
         out["mask_exact"] = bool(np.all(nh_gpu[:, mask == 0] == 0))
     print(json.dumps(out), flush=True)
 
+
+def _scipy_version():
+    try:
+        import scipy
+        return scipy.__version__
+    except Exception:
+        return "?"
+
+
+def _fftw_probe():
+    """Whether the host's dynamic linker knows a libfftw3 (ldconfig -p)."""
+    import subprocess
+    try:
+        out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:
+        return "unknown (ldconfig failed)"
+    libs = sorted({ln.split()[0] for ln in out.splitlines() if "libfftw3" in ln})
+    return libs or "none"
+
+
+def cpu_uxw_single(u_hat, m, kc, workers):
+    """The CPU baseline's u x omega term in single precision: the oracle's
+    steps (examples/HIST_CPU.jl:229-247: M, curl, inverse transforms, cross
+    product, forward transform, Leray projection, M, mean-free) on complex64 /
+    float32 arrays (scipy.fft transforms in the input's precision), the
+    derivative tables rounded to float32 and the FP64 mask predicate."""
+    import scipy.fft as sf
+    f32 = np.float32
+    mask = ((m.ksq() <= float(kc) * float(kc))).astype(f32)
+    a1, a2, a3 = (a.astype(f32) for a in (m.a1(), m.a2(), m.a3()))
+    s = m.grid.physical_shape
+    ut = mask[None] * u_hat
+    wt = np.stack([1j * (a2 * ut[2] - a3 * ut[1]), 1j * (a3 * ut[0] - a1 * ut[2]),
+                   1j * (a1 * ut[1] - a2 * ut[0])]).astype(np.complex64)
+    u = sf.irfftn(ut, s=s, axes=(-3, -2, -1), workers=workers)
+    w = sf.irfftn(wt, s=s, axes=(-3, -2, -1), workers=workers)
+    del ut, wt
+    nphys = np.stack([u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2],
+                      u[0] * w[1] - u[1] * w[0]])
+    del u, w
+    nt = sf.rfftn(nphys, axes=(-3, -2, -1), workers=workers)
+    del nphys
+    sq = (a1 * a1 + a2 * a2) + a3 * a3
+    lon = (a1 * nt[0] + a2 * nt[1] + a3 * nt[2]) / np.where(sq == 0, f32(1), sq)
+    out = np.stack([mask * (nt[0] - a1 * lon), mask * (nt[1] - a2 * lon), mask * (nt[2] - a3 * lon)])
+    out[:, 0, 0, 0] = 0
+    return out
 
 if __name__ == "__main__":
     main()

@@ -296,6 +296,7 @@ def to_physical_(dst: Field, src: Field, plans: FieldTransformPlans, scratch: Fi
         if scratch is src:
             raise AliasError(4, "scratch must not be the spectral source")
         _need(scratch, "spectral", src.order)
+        _with_plan(scratch, plans)
         sc = scratch.data
     check(load().gbg_to_physical(plans.handle, dst.data, src.data, src.ncomp, sc, _sh(stream)))
     return dst
@@ -693,8 +694,34 @@ class Server:
         self.handle = h.value
         self.inflight = {}
 
+    # kind -> (representation of in0, of out, tensor order or None = any, kept)
+    _IO = {0: ("physical", "spectral", None), 1: ("spectral", "physical", None),
+           2: ("spectral", "spectral", 1), 3: ("spectral", "spectral", None),
+           4: ("spectral", "spectral", 0), 5: ("spectral", "spectral", 1)}
+
     def submit(self, kind: int, out: Field, in0: Field, in1: Field | None = None,
                kc: float | None = None) -> int:
+        """Queue one job; the refusals of the matching standalone call apply
+        (representation, tensor order, grid and precision of every buffer;
+        no output that is an input, except the in-place curl and inverse
+        Laplacian)."""
+        if kind not in self._IO:
+            raise ArgumentError(1, f"unknown job kind {kind}")
+        rin, rout, order = self._IO[kind]
+        _need(in0, rin, order)
+        _need(out, rout, in0.order)
+        _same_domain(out, in0)
+        if kind == self.DEALIASED_PRODUCT:
+            if in1 is None:
+                raise ArgumentError(1, "the dealiased product takes two inputs")
+            _need(in1, "spectral", 0)
+            _same_domain(in1, in0)
+            if out is in1:
+                raise AliasError(4, "the product's output must not be an input")
+        elif in1 is not None:
+            raise ArgumentError(1, "this job kind takes one input")
+        if out is in0 and kind not in (self.CURL, self.INVERSE_LAPLACIAN):
+            raise AliasError(4, "this job kind cannot write over its input")
         g = in0.grid
         if kc is None:
             kc = min(dealias_cutoff(n, L) for n, L in zip(g.grid_size, g.domain_size))

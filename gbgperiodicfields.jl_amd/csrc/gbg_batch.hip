@@ -335,6 +335,20 @@ gbg_status gbg_batch_add(gbg_batch* b, int kind, const int n[3], const double L[
       it.n[a] = n[a];
       it.L[a] = L[a];
     }
+    {
+      // the refusals of gbg_gradient, gbg_uxw and gbg_dealiased_product: no
+      // output may overlap the input or the other output
+      const size_t cb = (size_t)(n[0] / 2 + 1) * n[1] * n[2] * 2 * elem_bytes;
+      const size_t nin = kind == GBG_ITEM_VECTOR_GRAD_UXW ? 3 : 1;
+      const char *i0 = (const char*)in, *o0 = (const char*)out0, *o1 = (const char*)out1;
+      auto ov = [](const char* a, size_t na, const char* c, size_t nc) {
+        return a < c + nc && c < a + na;
+      };
+      if (ov(o0, 3 * nin * cb, i0, nin * cb) || ov(o1, nin * cb, i0, nin * cb))
+        return fail(GBG_E_ALIAS, "a batch item's output overlaps its input");
+      if (ov(o0, 3 * nin * cb, o1, nin * cb))
+        return fail(GBG_E_ALIAS, "a batch item's two outputs overlap");
+    }
     it.elem = elem_bytes;
     it.in = in;
     it.out0 = out0;

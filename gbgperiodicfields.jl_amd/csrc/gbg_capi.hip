@@ -442,6 +442,8 @@ gbg_status gbg_to_physical(gbg_plan* p, void* phys, void* spec, int ncomp, void*
   void* work = scratch ? scratch : spec;
   if (scratch && overlaps(scratch, sb, spec, sb))
     return fail(GBG_E_ALIAS, "scratch overlaps the spectral source");
+  if (scratch && overlaps(scratch, sb, phys, ncomp * p->P * p->elem))
+    return fail(GBG_E_ALIAS, "scratch overlaps the physical destination");
   return inv_transform(p, phys, spec, work, ncomp, as_stream(stream));
   GBG_GUARD_END
 }

@@ -126,6 +126,22 @@ static size_t work_need(const gbg_job& j) {
   }
 }
 
+// Upper bound of a new plan's own device bytes before any work area, as
+// gbg_plan_create lays them out: per axis the two FP64 wavenumber tables and
+// the twiddle table, the reduction partials and results; plus one tile list
+// of the fused pipelines' last pass (at most two ints per output tile of at
+// least 8 pencils, one per (ky, kx) pencil bounds it).
+static size_t plan_fixed_bytes(const gbg_job& j) {
+  size_t b = sizeof(double) * kRedBlocks * 4 + sizeof(double) * 4;
+  for (int a = 0; a < 3; ++a) {
+    const size_t stored = a == 0 ? (size_t)(j.n[0] / 2 + 1) : (size_t)j.n[a];
+    b += 2 * sizeof(double) * stored + 2 * (size_t)j.elem_bytes * (size_t)j.n[a];
+  }
+  if (j.kind == GBG_JOB_UXW || j.kind == GBG_JOB_DEALIASED_PRODUCT)
+    b += (size_t)(j.n[0] / 2 + 1) * (size_t)j.n[1];
+  return b;
+}
+
 static size_t plan_bytes(const gbg_plan* p) { return p->device_bytes; }
 
 // Choose idle plans (least recently used first, never `keep`) whose release
@@ -150,6 +166,11 @@ static bool make_room(gbg_server* s, size_t need, const PlanKey& keep,
     s->plans.erase(victim);
   }
   return true;
+}
+
+static bool buffers_overlap(const void* a, size_t na, const void* b, size_t nb) {
+  const char *x = (const char*)a, *y = (const char*)b;
+  return x < y + nb && y < x + na;
 }
 
 static gbg_status run_job(gbg_plan* p, const gbg_job& j, hipStream_t st) {
@@ -193,10 +214,25 @@ static void dispatcher(gbg_server* s) {
     const bool fresh = it == s->plans.end();
     const size_t need = work_need(jb);
     const size_t have = fresh ? 0 : it->second.plan->work_bytes;
-    const size_t grow = fresh ? need + 65536 : (have < need ? need - have : 0);
+    const size_t grow = fresh ? need + plan_fixed_bytes(jb) : (have < need ? need - have : 0);
     std::vector<gbg_plan*> victims;
     if (grow && !make_room(s, grow, key, victims)) {
-      s->cv.wait(lk);  // admitted once a completion frees a plan
+      bool inflight = false;
+      for (int i = 0; i < s->slots; ++i) inflight = inflight || s->slot_busy[i];
+      if (inflight) {
+        s->cv.wait(lk);  // admitted once a completion frees a plan
+        continue;
+      }
+      // nothing runs and even every idle plan evicted leaves too little room
+      // (the job's own cached plan holds more than the budget allows beside
+      // its work area): no completion will ever make room, so the job ends
+      // with GBG_E_OOM instead of waiting
+      s->pending.pop_front();
+      j->status = GBG_E_OOM;
+      j->done = true;
+      j->dispatched = true;
+      j->t_done = Clock::now();
+      s->cv_done.notify_all();
       continue;
     }
     s->pending.pop_front();
@@ -292,7 +328,19 @@ gbg_status gbg_server_submit(gbg_server* s, const gbg_job* job, unsigned long lo
     if (nc != 1 && nc != 3 && nc != 9) return fail(GBG_E_DIM, "ncomp must be 1, 3 or 9");
     if ((j.kind == GBG_JOB_UXW || j.kind == GBG_JOB_DEALIASED_PRODUCT) && (!std::isfinite(j.kc) || j.kc < 0))
       return fail(GBG_E_ARG, "cutoff must be finite and >= 0");
-    if (work_need(j) + 65536 > s->budget)
+    // the buffers a job reads and writes (elements of one component)
+    const size_t M = (size_t)(j.n[0] / 2 + 1) * j.n[1] * j.n[2];
+    const size_t P = (size_t)j.n[0] * j.n[1] * j.n[2];
+    const size_t spec = (size_t)nc * M * 2 * j.elem_bytes, phys = (size_t)nc * P * j.elem_bytes;
+    const size_t in_b = j.kind == GBG_JOB_TO_SPECTRAL ? phys : spec;
+    const size_t out_b = j.kind == GBG_JOB_TO_PHYSICAL ? phys : spec;
+    const bool alias_ok = (j.kind == GBG_JOB_CURL || j.kind == GBG_JOB_INVERSE_LAPLACIAN) &&
+                          j.in0 == j.out;
+    if (!alias_ok && buffers_overlap(j.out, out_b, j.in0, in_b))
+      return fail(GBG_E_ALIAS, "the job's output overlaps its input");
+    if (j.kind == GBG_JOB_DEALIASED_PRODUCT && buffers_overlap(j.out, out_b, j.in1, in_b))
+      return fail(GBG_E_ALIAS, "the job's output overlaps its second input");
+    if (work_need(j) + plan_fixed_bytes(j) > s->budget)
       return fail(GBG_E_OOM, "job needs more device memory than the server's budget");
     ServeJob* sj = new ServeJob();
     sj->job = j;

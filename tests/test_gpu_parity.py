@@ -520,3 +520,36 @@ def test_refusals(G):
     a = G.Field(grid, "spectral", 0)
     with pytest.raises(G.AliasError):
         G.dealiased_product_(a, a, G.Field(grid, "spectral", 0), plans)
+    # to_physical! checks its scratch against the plans (src/Fields.jl:1601)
+    with pytest.raises(G.ArgumentError):
+        G.to_physical_(G.Field(grid, "physical", 1), s, plans,
+                       scratch=G.Field(G.Grid3D((8, 8, 4)), "spectral", 1))
+    with pytest.raises(G.ArgumentError):
+        G.to_physical_(G.Field(grid, "physical", 1), s, plans,
+                       scratch=G.Field(grid, "spectral", 1, np.float32))
+
+
+def test_c_abi_overlap_refusals(G):
+    """The C ABI refuses overlapping buffers on its own (no Python checks in
+    between): a to_physical scratch over the physical destination, and batch
+    items whose outputs overlap the input or each other."""
+    import ctypes as C
+    say("[refusals-c]")
+    L = G._lib.load()
+    grid = G.Grid3D((8, 8, 8))
+    plans = G.FieldTransformPlans(grid)
+    big = G.Field(grid, "spectral", 2)          # room for a vector field and more
+    spec = G.Field(grid, "spectral", 1)
+    st = L.gbg_to_physical(plans.handle, big.data, spec.data, 3, big.data + 64, None)
+    assert st == 4 and b"scratch overlaps the physical destination" in L.gbg_last_error()
+    b = G.Batch(nstreams=1, autotune=False)
+    n = (C.c_int * 3)(8, 8, 8)
+    Lb = (C.c_double * 3)(*grid.domain_size)
+    cb = 5 * 8 * 8 * 16
+    st = L.gbg_batch_add(b.handle, 0, n, Lb, 8, big.data, big.data + 16, big.data + 4 * cb, 2.0)
+    assert st == 4                              # out0 overlaps the input
+    st = L.gbg_batch_add(b.handle, 0, n, Lb, 8, big.data, big.data + cb, big.data + 16, 2.0)
+    assert st == 4                              # out1 overlaps the input
+    st = L.gbg_batch_add(b.handle, 0, n, Lb, 8, spec.data, big.data, big.data + cb, 2.0)
+    assert st == 4                              # out1 inside out0
+    b.close()

@@ -145,3 +145,101 @@ def test_serve_admission_and_eviction(G):
         b, k = srv.resident()
         assert b <= budget
     srv.close()
+
+
+def test_serve_submit_refusals(G):
+    """Server.submit applies the refusals of the matching standalone call;
+    the C ABI refuses an output that overlaps the input by itself."""
+    import ctypes as C
+    say("[serve] submit refusals")
+    S = G.Server
+    n = (16, 16, 16)
+    grid = G.Grid3D(n)
+    spec = G.Field(grid, "spectral", 1)
+    phys = G.Field(grid, "physical", 1)
+    srv = G.Server(hbm_budget_bytes=1 << 30, slots=1)
+    with pytest.raises(G.UnsupportedError):
+        srv.submit(S.TO_SPECTRAL, G.Field(grid, "spectral", 1), spec)        # needs a physical input
+    with pytest.raises(G.UnsupportedError):
+        srv.submit(S.TO_PHYSICAL, G.Field(grid, "spectral", 1), spec)       # needs a physical output
+    with pytest.raises(G.ArgumentError):
+        srv.submit(S.TO_SPECTRAL, G.Field(G.Grid3D((16, 16, 8)), "spectral", 1), phys)
+    with pytest.raises(G.ArgumentError):
+        srv.submit(S.CURL, G.Field(grid, "spectral", 1, np.float32), spec)
+    with pytest.raises(G.DimensionMismatch):
+        srv.submit(S.UXW, G.Field(grid, "spectral", 0), G.Field(grid, "spectral", 0))
+    with pytest.raises(G.DimensionMismatch):
+        srv.submit(S.TO_SPECTRAL, G.Field(grid, "spectral", 0), phys)
+    with pytest.raises(G.AliasError):
+        srv.submit(S.UXW, spec, spec)
+    a = G.Field(grid, "spectral", 0)
+    with pytest.raises(G.ArgumentError):
+        srv.submit(S.DEALIASED_PRODUCT, G.Field(grid, "spectral", 0), a)    # second input missing
+    with pytest.raises(G.AliasError):
+        srv.submit(S.DEALIASED_PRODUCT, a, G.Field(grid, "spectral", 0), a)
+    # the in-place curl is allowed, as curl! is alias-safe
+    t = srv.submit(S.CURL, spec, spec)
+    srv.wait(t)
+    # C ABI: an output overlapping the input
+    L = G._lib.load()
+    j = G._lib.GbgJob()
+    j.kind = S.UXW
+    j.n = (C.c_int * 3)(*n)
+    j.L = (C.c_double * 3)(*grid.domain_size)
+    j.elem_bytes = 8
+    j.ncomp = 3
+    j.in0 = spec.data
+    j.in1 = None
+    j.out = spec.data + 16
+    j.kc = 5.0
+    tk = C.c_ulonglong(0)
+    assert L.gbg_server_submit(srv.handle, C.byref(j), C.byref(tk)) == 4
+    srv.close()
+
+
+def test_serve_cached_plan_never_stalls(G):
+    """A job whose grid already has a cached plan is admitted against the
+    plan's real size: with a budget that holds the u x omega plan but not
+    more, a curl then a u x omega job on that grid both finish; with a budget
+    that cannot hold the plan beside the work area the u x omega job is
+    refused at submission (it never waits for a completion that cannot come).
+    The grid's long axes give the plan more than 64 KiB of tables."""
+    say("[serve] cached plan admission")
+    S = G.Server
+    n = (1024, 1024, 32)
+    grid = G.Grid3D(n)
+    g = O.Grid3D(n)
+    u = O.synthetic_field(g, 11, vector=True)
+    kc = O.dealias_cutoff(32)
+    du = G.Field(grid, "spectral", 1).upload(u)
+    outs = [G.Field(grid, "spectral", 1) for _ in range(2)]
+
+    def finish(srv, t, limit=60.0):
+        t0 = time.time()
+        while time.time() - t0 < limit:
+            lat = srv.poll(t)
+            if lat is not None:
+                return lat
+            time.sleep(0.001)
+        raise AssertionError("job never completed")
+
+    def resident_after(kind):
+        srv = G.Server(hbm_budget_bytes=64 << 30, slots=1)
+        finish(srv, srv.submit(kind, outs[0], du, kc=kc))
+        b, _ = srv.resident()
+        srv.close()
+        return b
+
+    r_curl, r_uxw = resident_after(S.CURL), resident_after(S.UXW)
+    assert r_curl > 64 * 1024 and r_uxw > r_curl
+    srv = G.Server(hbm_budget_bytes=r_uxw + (1 << 20), slots=1)
+    finish(srv, srv.submit(S.CURL, outs[0], du, kc=kc))
+    finish(srv, srv.submit(S.UXW, outs[1], du, kc=kc))
+    assert O.rel_l2(outs[1].download(), O.uxw(u, g, kc)[0]) < 1e-12
+    srv.close()
+    tight = G.Server(hbm_budget_bytes=(r_uxw - r_curl) + 64 * 1024, slots=1)
+    finish(tight, tight.submit(S.CURL, outs[0], du, kc=kc))
+    with pytest.raises(G.GbgError):
+        finish(tight, tight.submit(S.UXW, outs[1], du, kc=kc))
+    tight.close()
+
