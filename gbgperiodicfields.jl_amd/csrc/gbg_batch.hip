@@ -4,7 +4,9 @@
 // the register-resident pass (power-of-two lengths), and the mixed-radix LDS
 // pass under several radix factorisations/orderings and pencils-per-workgroup
 // widths -- on a scratch vector field and keeps the fastest (coordinate
-// descent over the three axes, forward + inverse transform as the score).
+// descent over the three axes, forward + inverse transform as the score);
+// then, on grids the fused pipelines cover, the x pass's tile shape (row-pair
+// units per workgroup), scored by one u x omega term.
 //
 // gbg_batch holds many independent items (a field on its own grid and its
 // operation), spreads them over a few streams (largest first, to the least
@@ -142,6 +144,34 @@ gbg_status autotune(gbg_plan* p, hipStream_t s, double* best_ms) {
     }
     apply(p, axis, st ? keep : win);
   }
+  // the fused pipelines' x-pass shape: one u x omega term (2/3 cutoff of the
+  // x axis) on a zero field in each shape, the faster kept
+  if (!st && is_fast_len(p->n[0]) && is_fast_len(p->n[1]) && is_fast_len(p->n[2]) &&
+      !p->generic_only) {
+    void* nh = nullptr;
+    st = check_hip(hipMalloc(&nh, sb), "tune buffer");
+    const double kc = ((2.0 / 3.0) * (p->n[0] / 2.0)) * (2.0 * M_PI / p->L[0]);
+    const int keep = p->x_var;
+    int win = keep;
+    double wbest = 1e30;
+    for (int v = 0; v < 2 && !st; ++v) {
+      p->x_var = v;
+      if ((st = pipe_uxw(p, spec, nh, kc, 0, s))) break;   // warm-up (work area)
+      if ((st = check_hip(hipEventRecord(e0, s), "tune event"))) break;
+      for (int i = 0; i < reps && !st; ++i) st = pipe_uxw(p, spec, nh, kc, 0, s);
+      if (st || (st = check_hip(hipEventRecord(e1, s), "tune event")) ||
+          (st = check_hip(hipEventSynchronize(e1), "tune sync")))
+        break;
+      float t = 0.f;
+      if ((st = check_hip(hipEventElapsedTime(&t, e0, e1), "tune elapsed"))) break;
+      if (t < wbest) {
+        wbest = t;
+        win = v;
+      }
+    }
+    p->x_var = st ? keep : win;
+    if (nh) (void)hipFree(nh);
+  }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (phys) (void)hipFree(phys);
@@ -245,6 +275,7 @@ static gbg_status instantiate(gbg_batch* b, hipStream_t user) {
             p->use_fast[a] = t->second->use_fast[a];
             p->bsel[a] = t->second->bsel[a];
           }
+          p->x_var = t->second->x_var;
         }
       }
       it.plan = p;

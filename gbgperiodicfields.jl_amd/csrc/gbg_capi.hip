@@ -192,6 +192,9 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
     p->generic_only = (env && env[0] == '1') ? 1 : 0;
     const char* wide = std::getenv("GBG_WIDE_OFFSETS");
     p->wide_offsets = (wide && wide[0] == '1') ? 1 : 0;
+    p->x_var = n[0] >= 1024 ? 1 : 0;   // the x pass's default shape (gbg_uxw.hip)
+    const char* xs = std::getenv("GBG_X_SHAPE");
+    if (xs && (xs[0] == '0' || xs[0] == '1') && xs[1] == 0) p->x_var = xs[0] - '0';
   }
   for (int a = 0; a < 3; ++a) {
     p->use_fast[a] = 1;

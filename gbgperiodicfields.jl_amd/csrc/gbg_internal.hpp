@@ -111,6 +111,10 @@ struct gbg_plan {
   // workgroup of the mixed-radix pass (0 = the default choice)
   int use_fast[3];
   int bsel[3];
+  // tile shape of the fused pipelines' x pass (0: the default row-pair units
+  // per workgroup, 1: half as many); chosen at creation from the x length,
+  // changed by gbg_plan_autotune or GBG_X_SHAPE=0|1 at creation
+  int x_var;
   // FP64 per-axis tables (device): derivative family and squared family
   double* kd[3];
   double* ks[3];

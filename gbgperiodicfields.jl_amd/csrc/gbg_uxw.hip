@@ -85,6 +85,24 @@ struct Geo {
 
 using fast::Cfg;
 
+// Tile shapes of the x pass (K3, the product's P3), chosen per plan
+// (gbg_plan::x_var): V = 0 the row-pair units of Cfg per workgroup; V = 1 half
+// as many, so more, smaller workgroups share a CU.  Measured per pass on the
+// MI355X (u x omega, 2/3 cutoff): at Nx = 1024 V = 1 is 5 % faster in FP64
+// (18.1 -> 17.1 ms) and FP32 (9.37 -> 8.87 ms); at Nx = 512 it is 9 % slower
+// in FP64 and within run-to-run spread in FP32.  (A larger staging area, and
+// for the z-inverse pass half the pencils per workgroup or sixteen elements
+// per thread, measured slower or spilled, and are not built.)
+template <typename T, int N, int V>
+struct XCfg {
+  using D = Cfg<T, N>;
+  static constexpr int XE = D::XE, XTP = D::XTP, XSYNC = D::XSYNC, XPD = D::XPD, XLD = D::XLD;
+  static constexpr int U = V == 1 ? (D::U > 1 ? D::U / 2 : 1) : D::U;
+  static constexpr int XNT = U * XTP;
+};
+// the x-pass shape a plan starts with (gbg_plan_autotune may change it)
+constexpr int x_default_shape(int n) { return n >= 1024 ? 1 : 0; }
+
 // With OPAQUE, the same pointer made opaque to the optimiser: table loads
 // behind it stay where they are written instead of being hoisted (and held in
 // registers) across the whole kernel.
@@ -575,7 +593,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 // PROD (the dealiased scalar product): the two fields a, b of a row share one
 // complex transform, p = a b is formed in registers, and p of the two rows of
 // the unit share one forward transform (one output field).
-template <typename T, int N>
+template <typename T, int N, int V>
 struct XStage {
   static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
   static constexpr int CH = sizeof(T) == 8 ? 3 : 2;            // copies per field row
@@ -590,13 +608,13 @@ struct XStage {
   static constexpr int TWN = ALL8 ? N / 2 : N;                 // twiddle entries the stages read
 };
 
-template <typename T, int N, bool STG, bool PROD>
-__global__ __launch_bounds__((Cfg<T, N>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
-                                                             cpx<T>* __restrict__ w3, Geo g,
-                                                             const cpx<T>* __restrict__ tw, T scale,
-                                                             long long ntiles) {
-  using C = Cfg<T, N>;
-  using X = XStage<T, N>;
+template <typename T, int N, bool STG, bool PROD, int V>
+__global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>* __restrict__ w2,
+                                                                 cpx<T>* __restrict__ w3, Geo g,
+                                                                 const cpx<T>* __restrict__ tw,
+                                                                 T scale, long long ntiles) {
+  using C = XCfg<T, N, V>;
+  using X = XStage<T, N, V>;
   constexpr int E = C::XE, TP = C::XTP, LD = C::XLD, S = C::XSYNC;
   constexpr int QOFF = PROD ? 1 : 3;   // field pairs (q, q + QOFF)
   static_assert(!STG || S, "staging needs a pencil per wave");
@@ -969,6 +987,59 @@ static bool narrow_offsets(const gbg_plan* p, const Geo& g) {
   return !p->wide_offsets && g.M < (1LL << 32) && g.s1 < (1LL << 32);
 }
 
+// K1; returns the launched grid (the partial-sum count)
+template <typename T, int N>
+static unsigned launch_z_inverse(const gbg_plan* p, const Geo& g, const cpx<T>* uh, cpx<T>* W1,
+                                 const Tables& tb, const cpx<T>* twz, double* part, hipStream_t s) {
+  using C = Cfg<T, N>;
+  const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
+  unsigned grid;
+  if (narrow_offsets(p, g)) {
+    grid = pass_grid(p, nt, z_inverse_persist<T, N, true>());
+    hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
+                       tb, twz, part, nt);
+  } else {
+    grid = pass_grid(p, nt, z_inverse_persist<T, N, false>());
+    hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
+                       tb, twz, part, nt);
+  }
+  return grid;
+}
+
+// a y pass (K2, K4, P2, P4)
+template <typename T, int N, int DIR, int NF>
+static void launch_y(const Geo& g, const cpx<T>* src, cpx<T>* dst, const Tables& tb,
+                     const cpx<T>* twy, hipStream_t s) {
+  using C = Cfg<T, N>;
+  const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
+  hipLaunchKernelGGL((k_uxw_y<T, N, DIR, NF>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, src,
+                     dst, g, tb, twy, nt);
+}
+
+// K3 / P3 in tile-shape variant V: the next field pair is staged through LDS
+// when a row sits in one wave and the kept kx range fits the staging area
+template <typename T, int N, bool PROD, int V>
+static void launch_x_v(const Geo& g, const cpx<T>* W2, cpx<T>* W1, const cpx<T>* twx, T scale,
+                       hipStream_t s) {
+  using C = XCfg<T, N, V>;
+  const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
+  const long long nt = (pairs + C::U - 1) / C::U;
+  if (C::XSYNC && g.nhk <= XStage<T, N, V>::CAP)
+    hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0, PROD, V>), dim3(fast::xcd_grid(nt)),
+                       dim3(C::XNT), 0, s, W2, W1, g, twx, scale, nt);
+  else
+    hipLaunchKernelGGL((k_uxw_x<T, N, false, PROD, V>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
+                       s, W2, W1, g, twx, scale, nt);
+}
+
+template <typename T, int N, bool PROD>
+static void launch_x(int var, const Geo& g, const cpx<T>* W2, cpx<T>* W1, const cpx<T>* twx, T scale,
+                     hipStream_t s) {
+  if constexpr (N >= 256)
+    if (var == 1) return launch_x_v<T, N, PROD, 1>(g, W2, W1, twx, scale, s);
+  launch_x_v<T, N, PROD, 0>(g, W2, W1, twx, scale, s);
+}
+
 // K5's transform kernel over nt tiles (tile list `list`, or all tiles)
 template <typename T, int N, int NC>
 static void launch_z_forward(const gbg_plan* p, const Geo& g, long long nt, const cpx<T>* w,
@@ -1213,54 +1284,24 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   prof_begin(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
-    unsigned grid;
-    if (narrow_offsets(p, g)) {
-      grid = pass_grid(p, nt, z_inverse_persist<T, N, true>());
-      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
-                         tb, twz, part, nt);
-    } else {
-      grid = pass_grid(p, nt, z_inverse_persist<T, N, false>());
-      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
-                         tb, twz, part, nt);
-    }
+    const unsigned grid = launch_z_inverse<T, N>(p, g, uh, W1, tb, twz, part, s);
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
   });
   if ((st = check_launch("uxw K1"))) return st;
   const bool forked = zero_tiles_launch<T>(p, g, fork, 3, out);
   prof_mark(p, s);
   with_len(g.ny, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, 1, 6>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
-                       tb, twy, nt);
+    launch_y<T, decltype(L)::value, 1, 6>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("uxw K2"))) return st;
   prof_mark(p, s);
   with_len(g.nx, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
-    const long long nt = (pairs + C::U - 1) / C::U;
-    // stage the next pair through LDS when a pencil sits in one wave and the
-    // kept kx range fits the staging area
-    if (C::XSYNC && g.nhk <= XStage<T, N>::CAP)
-      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
-                         s, W2, W1, g, twx, T(1.0 / P), nt);
-    else
-      hipLaunchKernelGGL((k_uxw_x<T, N, false, false>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0, s, W2,
-                         W1, g, twx, T(1.0 / P), nt);
+    launch_x<T, decltype(L)::value, false>(p->x_var, g, W2, W1, twx, T(1.0 / P), s);
   });
   if ((st = check_launch("uxw K3"))) return st;
   prof_mark(p, s);
   with_len(g.ny, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, -1, 3>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2, g,
-                       tb, twy, nt);
+    launch_y<T, decltype(L)::value, -1, 3>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("uxw K4"))) return st;
   prof_mark(p, s);
@@ -1368,34 +1409,17 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
   const bool forked = zero_tiles_launch<T>(p, g, fork, 1, out);
   prof_mark(p, s);
   with_len(g.ny, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, 1, 2>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2,
-                       g, tb, twy, nt);
+    launch_y<T, decltype(L)::value, 1, 2>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("product P2"))) return st;
   prof_mark(p, s);
   with_len(g.nx, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long pairs = ((long long)g.ny * g.nz + 1) / 2;
-    const long long nt = (pairs + C::U - 1) / C::U;
-    if (C::XSYNC && g.nhk <= XStage<T, N>::CAP)
-      hipLaunchKernelGGL((k_uxw_x<T, N, C::XSYNC != 0, true>), dim3(fast::xcd_grid(nt)),
-                         dim3(C::XNT), 0, s, W2, W1, g, twx, T(1.0 / P), nt);
-    else
-      hipLaunchKernelGGL((k_uxw_x<T, N, false, true>), dim3(fast::xcd_grid(nt)), dim3(C::XNT), 0,
-                         s, W2, W1, g, twx, T(1.0 / P), nt);
+    launch_x<T, decltype(L)::value, true>(p->x_var, g, W2, W1, twx, T(1.0 / P), s);
   });
   if ((st = check_launch("product P3"))) return st;
   prof_mark(p, s);
   with_len(g.ny, [&](auto L) {
-    constexpr int N = decltype(L)::value;
-    using C = Cfg<T, N>;
-    const long long nt = ((long long)g.nz * g.nhp + C::B - 1) / C::B;
-    hipLaunchKernelGGL((k_uxw_y<T, N, -1, 1>), dim3(fast::xcd_grid(nt)), dim3(C::NT), 0, s, W1, W2,
-                       g, tb, twy, nt);
+    launch_y<T, decltype(L)::value, -1, 1>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("product P4"))) return st;
   prof_mark(p, s);

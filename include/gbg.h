@@ -208,8 +208,11 @@ gbg_status gbg_dealiased_product(gbg_plan* plan, void* p_hat, const void* a_hat,
  * Times, per axis, the candidate pass configurations (register-resident pass
  * for power-of-two lengths; mixed-radix LDS pass under several radix
  * orderings and pencils-per-workgroup widths) on a scratch vector field and
- * keeps the fastest in the plan.  Allocates and frees scratch, synchronises
- * the stream.  *best_ms (may be NULL) = forward + inverse time of the choice.
+ * keeps the fastest in the plan; on power-of-two grids it then times the
+ * fused pipelines' x pass in each tile shape (row-pair units per workgroup)
+ * over one u x omega term and keeps the faster.  Allocates and frees
+ * scratch, synchronises the stream.  *best_ms (may be NULL) = forward +
+ * inverse time of the transform choice.
  * No counterpart in the reference (FFTW's planner flags play this role,
  * src/Fields.jl:1352-1362). */
 gbg_status gbg_plan_autotune(gbg_plan* plan, void* stream, double* best_ms);

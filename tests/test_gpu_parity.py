@@ -274,6 +274,43 @@ def test_uxw_boxes_and_tiny_cutoffs(G, n, L, kc, dtype):
         assert e == 0.0 and z == 0.0
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(256, 64, 64), (512, 32, 64), (1024, 32, 32)])
+def test_x_pass_shapes(G, n, dtype, monkeypatch):
+    """The x pass in both tile shapes (row-pair units per workgroup, forced
+    with GBG_X_SHAPE at plan creation): bitwise the same N^ and product, and
+    the oracle's; the autotuner keeps results unchanged."""
+    say(f"[x-shapes] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    u = O.synthetic_field(g, 2004, vector=True, m=m)
+    a = O.synthetic_field(g, 1005, m=m)
+    kc = O.dealias_cutoff(n[0])
+    ref = O.uxw(u, g, kc, m)[0]
+    pref = O.dealiased_product(a, a, g, kc, m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    da = G.Field(grid, "spectral", 0, dtype).upload(a)
+    outs = []
+    for shape in ("0", "1", "tuned"):
+        if shape == "tuned":
+            plans = G.FieldTransformPlans(grid, dtype)
+            G.autotune(plans)
+        else:
+            monkeypatch.setenv("GBG_X_SHAPE", shape)
+            plans = G.FieldTransformPlans(grid, dtype)
+            monkeypatch.delenv("GBG_X_SHAPE")
+        dn = G.Field(grid, "spectral", 1, dtype)
+        G.uxw_(dn, du, plans, kc)
+        dp = G.Field(grid, "spectral", 0, dtype)
+        G.dealiased_product_(dp, da, da, plans, kc)
+        outs.append((dn.download(), dp.download()))
+        assert O.rel_l2(outs[-1][0], ref) < TOL[dtype]
+        assert O.rel_l2(outs[-1][1], pref) < TOL[dtype]
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+
+
 def _fp32_mask_case():
     n, L = (64, 64, 64), (1.3, 1.3, 1.3)
     g = ogrid(n, L)
