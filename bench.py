@@ -129,29 +129,33 @@ def uxw_bytes_unpruned(n, elem):
     return 42 * M * 2 * elem
 
 
-KERNEL_OF_STAGE = {"K1_z_inverse_curl": "k_uxw_z_inverse<{t}, {n}, true>",
-                   "K2_y_inverse": "k_uxw_y<{t}, {n}, 1, 6>",
-                   "K3_x_product": "k_uxw_x<{t}, {n}, true, false>",
-                   "K4_y_forward": "k_uxw_y<{t}, {n}, -1, 3>",
-                   "K5_z_forward_project": ("k_uxw_zero_tiles<{t}>", "k_uxw_z_forward<{t}, {n}, 3>")}
+KERNEL_OF_STAGE = {"K1_z_inverse_curl": ("k_uxw_z_inverse<{t}, {n},",),
+                   "K2_y_inverse": ("k_uxw_y<{t}, {n}, 1, 6",),
+                   "K3_x_product": ("k_uxw_x<{t}, {n},",),
+                   "K4_y_forward": ("k_uxw_y<{t}, {n}, -1, 3",),
+                   "K5_z_forward_project": ("k_uxw_zero_tiles<{t}>", "k_uxw_z_forward<{t}, {n}, 3")}
 
 
 def measured_traffic(stage, n, dtype):
     """HBM bytes per launch of a pass, from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs (copy-kernel calibrated), or None if none match."""
+    WRITE_SIZE runs (copy-kernel calibrated), or None if none match.  Each
+    prefix of the pass names one kernel of the profiled run (its template
+    arguments up to the ones that name the shape)."""
     path = os.path.join(REPO, "profiles", f"uxw_{n[0]}_hbm_traffic.json")
     if not os.path.exists(path) or len(set(n)) != 1:
         return None
     with open(path) as fh:
         data = json.load(fh)
     t = "double" if dtype == np.float64 else "float"
-    ks = KERNEL_OF_STAGE[stage]
+    kernels = data.get("kernels", {})
     total = 0
-    for k in (ks,) if isinstance(ks, str) else ks:
-        rec = data.get("kernels", {}).get(k.format(t=t, n=n[0]))
-        if rec is None:
+    for pre in KERNEL_OF_STAGE[stage]:
+        pre = pre.format(t=t, n=n[0])
+        hits = [k for k in kernels if k.startswith(pre) and
+                (pre[-1] in ",>" or k[len(pre):len(pre) + 1] in (",", ">"))]
+        if len(hits) != 1:
             return None
-        total += int(rec["hbm_bytes"])
+        total += int(kernels[hits[0]]["hbm_bytes"])
     return total
 
 
