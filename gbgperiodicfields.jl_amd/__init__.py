@@ -32,6 +32,7 @@ __all__ = [
     "Grid3D", "FieldTransformPlans", "Field", "ScalarFieldPhysical", "VectorFieldPhysical",
     "ScalarFieldSpectral", "VectorFieldSpectral", "TensorFieldSpectral", "Stream",
     "to_spectral_", "to_physical_", "to_spectral", "to_physical", "gradient_", "gradient",
+    "symmetric_gradient_", "symmetric_gradient",
     "divergence_", "curl_", "curl", "inverse_curl_", "laplacian_", "inverse_laplacian_",
     "helmholtz_projection_", "spectral_cutoff", "spectral_band", "convolve_",
     "mean_free_projection_", "cross_", "multiply_", "inner", "l2norm", "uxw_",
@@ -330,6 +331,20 @@ def gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -
 
 def gradient(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
     return gradient_(Field(src.grid, "spectral", src.order + 1, src.dtype), src, plans, stream)
+
+
+def symmetric_gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    """``symmetric_gradient!`` vector->symmetric tensor (``:1289-1300``)."""
+    _need(src, "spectral", 1)
+    _need(dst, "spectral", 2)
+    _with_plan(src, plans)
+    _same_domain(dst, src)
+    check(load().gbg_symmetric_gradient(plans.handle, dst.data, src.data, _sh(stream)))
+    return dst
+
+
+def symmetric_gradient(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
+    return symmetric_gradient_(Field(src.grid, "spectral", 2, src.dtype), src, plans, stream)
 
 
 def divergence_(dst: Field, src: Field, plans: FieldTransformPlans, kernel=None, alpha=1.0,

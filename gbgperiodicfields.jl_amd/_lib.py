@@ -54,6 +54,7 @@ SIGNATURES = {
     "gbg_to_spectral": (st, [vp, vp, vp, i32, vp]),
     "gbg_to_physical": (st, [vp, vp, vp, i32, vp, vp]),
     "gbg_gradient": (st, [vp, vp, vp, i32, vp]),
+    "gbg_symmetric_gradient": (st, [vp, vp, vp, vp]),
     "gbg_divergence": (st, [vp, vp, vp, i32, i32, dbl, dbl, dbl, dbl, vp]),
     "gbg_curl": (st, [vp, vp, vp, vp]),
     "gbg_inverse_curl": (st, [vp, vp, vp, vp]),

@@ -200,6 +200,49 @@ __device__ cpx<T>* fft_lds(cpx<T>* s0, cpx<T>* s1, int ld, const AxisPlan& ap, i
   return a;
 }
 
+// Whole-pencil transform of any axis plan: the mixed-radix form above, or for
+// a chirp-z axis (ap.m > 0) Bluestein's convolution
+//   X_k = c_k sum_j (x_j c_j) conj(c_{k-j}),  c_j = e^{DIR i pi j^2 / n},
+// carried out in LDS: the n entries of each pencil times the chirps, zeros up
+// to m, a forward m-point transform, the filter spectrum (1/m included), an
+// inverse m-point transform, and the chirps again on the first n entries.  The
+// pencils in s0 hold n entries (stride ld >= m + 1) and a barrier separates
+// their writes from this call; the result (n entries) is in the returned buffer.
+template <typename T, int DIR>
+__device__ cpx<T>* fft_any(cpx<T>* s0, cpx<T>* s1, int ld, const AxisPlan& ap, int B,
+                           const cpx<T>* __restrict__ tw) {
+  if (ap.m == 0) return fft_lds<T, DIR>(s0, s1, ld, ap, B, tw);
+  const int n = ap.n, m = ap.m;
+  const cpx<T>* chirp = tw + m;
+  const cpx<T>* filt = tw + m + n + (DIR > 0 ? m : 0);
+  auto c_at = [&](int j) {
+    cpx<T> c = chirp[j];
+    if (DIR > 0) c.y = -c.y;
+    return c;
+  };
+  for (int idx = threadIdx.x; idx < B * m; idx += blockDim.x) {
+    const int p = idx / m, e = idx - p * m;
+    cpx<T>* v = s0 + p * ld + e;
+    *v = e < n ? cmul(*v, c_at(e)) : mk<T>(T(0), T(0));
+  }
+  __syncthreads();
+  AxisPlan am = ap;
+  am.n = m;
+  cpx<T>* r = fft_lds<T, -1>(s0, s1, ld, am, B, tw);
+  for (int idx = threadIdx.x; idx < B * m; idx += blockDim.x) {
+    const int p = idx / m, k = idx - p * m;
+    r[p * ld + k] = cmul(r[p * ld + k], filt[k]);
+  }
+  __syncthreads();
+  cpx<T>* y = fft_lds<T, 1>(r, r == s0 ? s1 : s0, ld, am, B, tw);
+  for (int idx = threadIdx.x; idx < B * n; idx += blockDim.x) {
+    const int p = idx / n, k = idx - p * n;
+    y[p * ld + k] = cmul(y[p * ld + k], c_at(k));
+  }
+  __syncthreads();
+  return y;
+}
+
 // LDS of the generic pass kernels: two buffers (ping-pong) of 64 KiB payload
 // each plus one padding element per pencil (B <= 64).
 template <typename T>

@@ -61,10 +61,12 @@ static std::vector<AxisChoice> candidates(gbg_plan* p, int axis) {
   base.use_fast = 1;
   base.bsel = 0;
   if (is_fast_len(n)) c.push_back(base);  // register-resident pass
+  // a chirp-z axis: the orders of its convolution length's radices
+  const int len = pencil_len(base.ap);
   std::vector<std::vector<int>> orders;
   std::vector<int> f;
   for (int big : {8, 4}) {
-    if (!factor_with(n, big, f)) continue;
+    if (!factor_with(len, big, f)) continue;
     orders.push_back(f);
     std::vector<int> rev(f.rbegin(), f.rend());
     orders.push_back(rev);
@@ -75,12 +77,13 @@ static std::vector<AxisChoice> candidates(gbg_plan* p, int axis) {
   std::sort(orders.begin(), orders.end());
   orders.erase(std::unique(orders.begin(), orders.end()), orders.end());
   const int cap = 65536 / (2 * p->elem);
-  const int b0 = pick_b(n, cap);
+  const int b0 = pick_b(len, cap);
   for (const auto& o : orders) {
     for (int b : {b0, b0 / 2, b0 / 4}) {
       if (b < 1) continue;
       AxisChoice ch;
       ch.ap.n = n;
+      ch.ap.m = base.ap.m;
       ch.ap.nst = (int)o.size();
       for (int i = 0; i < ch.ap.nst; ++i) ch.ap.radix[i] = o[i];
       ch.use_fast = 0;

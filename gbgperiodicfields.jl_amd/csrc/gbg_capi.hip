@@ -30,20 +30,33 @@ gbg_status check_hip(hipError_t e, const char* what) {
 
 gbg_status check_launch(const char* what) { return check_hip(hipGetLastError(), what); }
 
-bool factor_axis(int n, AxisPlan* ap) {
-  ap->n = n;
+static bool factor_into(int n, AxisPlan* ap) {
   ap->nst = 0;
   int m = n;
   const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
   for (int r : order) {
     while (m % r == 0) {
-      if (r == 8 && m % 8 != 0) break;
       if (ap->nst >= kMaxStages) return false;
       ap->radix[ap->nst++] = r;
       m /= r;
     }
   }
   return m == 1;
+}
+
+bool factor_axis(int n, AxisPlan* ap) {
+  ap->n = n;
+  ap->m = 0;
+  if (factor_into(n, ap)) return true;
+  // chirp-z: the smallest 2^a 3^b 5^c >= 2n - 1
+  long long best = 0;
+  for (long long a = 1; a < (1LL << 40); a *= 2)
+    for (long long b = a; b < (1LL << 40); b *= 3)
+      for (long long c = b; c < (1LL << 40); c *= 5)
+        if (c >= 2LL * n - 1 && (!best || c < best)) best = c;
+  if (!best || best > (1 << 30)) return false;
+  ap->m = (int)best;
+  return factor_into(ap->m, ap);
 }
 
 gbg_status ensure_work(gbg_plan* p, size_t bytes) {
@@ -109,14 +122,54 @@ static void make_twiddles(int n, std::vector<long double>& re, std::vector<long 
   }
 }
 
+// The chirp-z tables of an axis of length n over convolution length m
+// (Bluestein's algorithm): with w_j = e^{-i pi j^2 / n},
+//   X_k = w_k sum_j (x_j w_j) conj(w_{k-j}),
+// a cyclic convolution of length m >= 2n - 1 of a_j = x_j w_j (zero past n)
+// with b_j = conj(w_j) for |j| < n (b at -j stored at m - j).  Appended to the
+// m-entry twiddle table: the n chirps w_j, then B = (1/m) DFT_m(b) for the
+// forward transform and the same for the inverse, whose chirps are conj(w_j).
+// w_j = e^{-2 pi i (j^2 mod 2n) / (2n)} from the exact-octant table of 2n.
+static void chirp_tables(int n, int m, std::vector<long double>& re, std::vector<long double>& im) {
+  std::vector<long double> cr, ci, mr, mi;
+  make_twiddles(2 * n, cr, ci);
+  make_twiddles(m, mr, mi);
+  std::vector<long double> wr(n), wi(n);
+  for (int j = 0; j < n; ++j) {
+    const long long q = ((long long)j * j) % (2LL * n);
+    wr[j] = cr[q];
+    wi[j] = ci[q];
+  }
+  re.insert(re.end(), wr.begin(), wr.end());
+  im.insert(im.end(), wi.begin(), wi.end());
+  for (int dir = 0; dir < 2; ++dir) {
+    // b_j = conj(w_j) forward, w_j inverse, on j in (-n, n), wrapped mod m
+    const long double sg = dir == 0 ? -1.0L : 1.0L;
+    for (int k = 0; k < m; ++k) {
+      long double sr = 0.0L, si = 0.0L;
+      for (int j = -(n - 1); j <= n - 1; ++j) {
+        const int a = j < 0 ? -j : j;
+        const long double br = wr[a], bi = sg * wi[a];
+        const long long e = (((long long)(j + m) % m) * k) % m;   // e^{-2 pi i j k / m}
+        sr += br * mr[e] - bi * mi[e];
+        si += br * mi[e] + bi * mr[e];
+      }
+      re.push_back(sr / m);
+      im.push_back(si / m);
+    }
+  }
+}
+
 template <typename T>
 static gbg_status upload_twiddles(gbg_plan* p, int axis) {
   std::vector<long double> re, im;
-  const int n = p->n[axis];
-  make_twiddles(n, re, im);
-  std::vector<cpx<T>> h(n);
-  for (int m = 0; m < n; ++m) h[m] = mk<T>((T)(double)re[m], (T)(double)im[m]);
-  const size_t bytes = sizeof(cpx<T>) * (size_t)n;
+  const AxisPlan& ap = p->ax[axis];
+  make_twiddles(pencil_len(ap), re, im);
+  if (ap.m) chirp_tables(ap.n, ap.m, re, im);
+  const size_t cnt = re.size();
+  std::vector<cpx<T>> h(cnt);
+  for (size_t m = 0; m < cnt; ++m) h[m] = mk<T>((T)(double)re[m], (T)(double)im[m]);
+  const size_t bytes = sizeof(cpx<T>) * cnt;
   gbg_status st = check_hip(hipMalloc(&p->tw[axis], bytes), "twiddle table");
   if (st) return st;
   p->device_bytes += bytes;
@@ -176,14 +229,19 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
     p->L[a] = L[a];
     if (!factor_axis(n[a], &p->ax[a])) {
       destroy_plan(p);
-      return fail(GBG_E_UNSUPPORTED, "axis length " + std::to_string(n[a]) +
-                                         " has a prime factor above 13 (supported radices 2,3,4,5,7,8,11,13)");
+      return fail(GBG_E_UNSUPPORTED, "axis length " + std::to_string(n[a]) + " cannot be factored");
     }
-    // the LDS pencil kernels hold at least one whole pencil per workgroup
+    // the LDS pencil kernels hold at least one whole pencil per workgroup (a
+    // chirp-z axis: one pencil of its convolution length)
     const int cap = 65536 / (2 * elem);
-    if (n[a] + 1 > cap) {
+    if (pencil_len(p->ax[a]) + 1 > cap) {
       destroy_plan(p);
-      return fail(GBG_E_UNSUPPORTED, "axis length " + std::to_string(n[a]) + " exceeds one LDS pencil");
+      return fail(GBG_E_UNSUPPORTED,
+                  "axis length " + std::to_string(n[a]) +
+                      (p->ax[a].m ? " (a prime factor above 13: chirp-z length " +
+                                        std::to_string(p->ax[a].m) + ")"
+                                  : std::string()) +
+                      " exceeds one LDS pencil");
     }
   }
   p->elem = elem;
@@ -462,6 +520,18 @@ gbg_status gbg_gradient(gbg_plan* p, void* dst, const void* src, int src_order, 
   if (overlaps(dst, 3 * nsrc * cb, src, nsrc * cb))
     return fail(GBG_E_ALIAS, "gradient raises the order and cannot alias");
   return op_gradient(p, dst, src, src_order, as_stream(stream));
+  GBG_GUARD_END
+}
+
+gbg_status gbg_symmetric_gradient(gbg_plan* p, void* dst, const void* src, void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!dst || !src) return fail(GBG_E_ARG, "null field buffer");
+  const size_t cb = p->M * 2 * p->elem;
+  if (overlaps(dst, 9 * cb, src, 3 * cb))
+    return fail(GBG_E_ALIAS, "symmetric gradient raises the order and cannot alias");
+  return op_symmetric_gradient(p, dst, src, as_stream(stream));
   GBG_GUARD_END
 }
 

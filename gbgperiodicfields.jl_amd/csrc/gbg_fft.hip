@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kNT) void k_c2c_strided(const cpx<T>* in, cpx<T>* o
     s[pl * d.ld + e] = v;
   }
   __syncthreads();
-  const cpx<T>* r = fft_lds<T, DIR>(s, s + Lds<T>::kElems, d.ld, ap, d.B, tw);
+  const cpx<T>* r = fft_any<T, DIR>(s, s + Lds<T>::kElems, d.ld, ap, d.B, tw);
   if (valid) {
     for (int e = e0; e < n; e += estep) out[base + (long long)e * d.se] = cscale(r[pl * d.ld + e], scale);
   }
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kNT) void k_r2c_x(const T* __restrict__ in, cpx<T>*
     s[q * ld + x] = mk<T>(a, b);
   }
   __syncthreads();
-  const cpx<T>* rs = fft_lds<T, -1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  const cpx<T>* rs = fft_any<T, -1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
   for (int idx = threadIdx.x; idx < B * nh; idx += kNT) {
     const int q = idx / nh, k = idx - q * nh;
     const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kNT) void k_c2r_x(const cpx<T>* __restrict__ in, T*
     s[q * ld + k] = mk<T>(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
-  const cpx<T>* rs = fft_lds<T, 1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  const cpx<T>* rs = fft_any<T, 1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
   for (int idx = threadIdx.x; idx < B * nx; idx += kNT) {
     const int q = idx / nx, x = idx - q * nx;
     const long long r0 = 2 * (pair0 + q), r1 = r0 + 1;
@@ -374,9 +374,9 @@ static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* i
     d.se = ny * nh;
   }
   d.cstride = (long long)p->M;
-  d.B = p->bsel[axis] ? p->bsel[axis] : pick_b(ap.n, Lds<T>::kCap);
+  d.B = p->bsel[axis] ? p->bsel[axis] : pick_b(pencil_len(ap), Lds<T>::kCap);
   d.bshift = ilog2(d.B);
-  d.ld = ap.n + 1;
+  d.ld = pencil_len(ap) + 1;
   const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(p->tw[axis]);
   if (!p->generic_only && p->use_fast[axis] &&
       try_fast_c2c<T>(ap.n, dir, in, out, d, ncomp, tw, scale, p->ncu, s))
@@ -397,10 +397,11 @@ static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, 
   const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
   if (p->generic_only || !p->use_fast[0] ||
       !try_fast_x<T>(nx, -1, phys, spec, nrows, rg, twx, T(1), s)) {
-    const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
+    const int len = pencil_len(p->ax[0]);
+    const int B = p->bsel[0] ? p->bsel[0] : pick_b(len, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
     dim3 grid(fast::xcd_grid((pairs + B - 1) / B));
-    hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, nx + 1,
+    hipLaunchKernelGGL((k_r2c_x<T>), grid, dim3(kNT), 0, s, phys, spec, nrows, nx, nh, B, len + 1,
                        p->ax[0], twx);
   }
   gbg_status st = check_launch("r2c x pass");
@@ -422,10 +423,11 @@ static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, 
   const RowG rg{nh, nh, nx % 2 == 0 ? nh - 1 : -1};
   if (p->generic_only || !p->use_fast[0] ||
       !try_fast_x<T>(nx, 1, work, phys, nrows, rg, twx, scale, s)) {
-    const int B = p->bsel[0] ? p->bsel[0] : pick_b(nx, Lds<T>::kCap);
+    const int len = pencil_len(p->ax[0]);
+    const int B = p->bsel[0] ? p->bsel[0] : pick_b(len, Lds<T>::kCap);
     const long long pairs = (nrows + 1) / 2;
     dim3 grid(fast::xcd_grid((pairs + B - 1) / B));
-    hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, nx + 1,
+    hipLaunchKernelGGL((k_c2r_x<T>), grid, dim3(kNT), 0, s, work, phys, nrows, nx, nh, B, len + 1,
                        p->ax[0], twx, scale);
   }
   return check_launch("c2r x pass");

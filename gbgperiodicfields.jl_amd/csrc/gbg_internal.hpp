@@ -57,11 +57,21 @@ struct AxisPlan {
   int n;
   int nst;
   int radix[kMaxStages];
+  // 0: radix[] factors n (a direct mixed-radix transform).  Otherwise n has a
+  // prime factor above 13 and the axis is transformed as a chirp-z
+  // convolution of length m (Bluestein): radix[] factors m, and the axis's
+  // twiddle table holds the m-entry table, then the n chirps e^{-i pi j^2/n},
+  // then the m-point spectra of the forward and inverse chirp filters.
+  int m;
 };
 
 // Factor n into the radices the butterflies implement (8, 4, 2, 3, 5, 7, 11,
-// 13).  Returns false when n has a larger prime factor.
+// 13); when n has a larger prime factor, set up the chirp-z form over the
+// smallest m >= 2n - 1 of the form 2^a 3^b 5^c.  Returns false only when no
+// factorisation fits kMaxStages.
 bool factor_axis(int n, AxisPlan* ap);
+// LDS entries one pencil of the axis occupies: n, or m for a chirp-z axis
+inline int pencil_len(const AxisPlan& ap) { return ap.m ? ap.m : ap.n; }
 
 // FP64 per-axis wavenumber tables (device pointers): derivative family kd*,
 // squared family ks*.
@@ -118,7 +128,8 @@ struct gbg_plan {
   // FP64 per-axis tables (device): derivative family and squared family
   double* kd[3];
   double* ks[3];
-  // twiddle tables, n entries e^{-2 pi i m / n} in the plan's precision
+  // twiddle tables, n entries e^{-2 pi i m / n} in the plan's precision (a
+  // chirp-z axis: its m-entry table and chirp tables, AxisPlan::m)
   void* tw[3];
   // pipeline work area (allocated on first use, kept)
   void* work;
@@ -170,6 +181,7 @@ gbg_status inv_transform(gbg_plan* p, void* phys, void* spec_in, void* spec_work
 
 // operators (gbg_ops.hip)
 gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, hipStream_t s);
+gbg_status op_symmetric_gradient(gbg_plan* p, void* dst, const void* src, hipStream_t s);
 gbg_status op_divergence(gbg_plan* p, void* dst, const void* src, int src_order, int has_band,
                          double lo, double hi, double alpha, double beta, hipStream_t s);
 gbg_status op_curl(gbg_plan* p, void* dst, const void* src, int inverse, hipStream_t s);

@@ -107,6 +107,48 @@ __global__ __launch_bounds__(kOpNT) void k_gradient_pair(float4* __restrict__ ds
   }
 }
 
+// ---- symmetric gradient --------------------------------------------------------
+// vector -> symmetric tensor (_symmetric_gradient_kernel!, :666-699):
+//   S_ii = (i a_i) u_i,  S_ij = S_ji = (i/2) (a_j u_i + a_i u_j)
+// with (i/2) the complex (0, 1/2) multiplying the sum.  P consecutive modes
+// per lane (P = 2 in FP32: 16-byte accesses), the three sources read once and
+// the six distinct values written to the nine blocks (block i + 3 j).
+template <typename T, int P>
+struct alignas(P * sizeof(cpx<T>)) ModePack {
+  cpx<T> c[P];
+};
+
+template <typename T, int P>
+__global__ __launch_bounds__(kOpNT) void k_symmetric_gradient(ModePack<T, P>* __restrict__ dst,
+                                                              const ModePack<T, P>* __restrict__ src,
+                                                              unsigned MP, ModeIdx mi, Tables t) {
+  const unsigned idx = blockIdx.x * kOpNT + threadIdx.x;
+  if (idx >= MP) return;
+  ModePack<T, P> u[3], d[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) u[i] = src[(size_t)i * MP + idx];
+  const cpx<T> half_i = mk<T>(T(0), T(0.5));
+#pragma unroll
+  for (int h = 0; h < P; ++h) {
+    unsigned kx, ky, kz;
+    mi(P * idx + h, kx, ky, kz);
+    const T a[3] = {T(t.kd1[kx]), T(t.kd2[ky]), T(t.kd3[kz])};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i].c[h] = cmul(mk<T>(T(0), a[i]), u[i].c[h]);
+    // d[3] = S_12, d[4] = S_13, d[5] = S_23
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = q == 2 ? 1 : 0, j = q == 0 ? 1 : 2;
+      const cpx<T> sum = cadd(cscale(u[i].c[h], a[j]), cscale(u[j].c[h], a[i]));
+      d[3 + q].c[h] = cmul(half_i, sum);
+    }
+  }
+  // blocks: (1,1) 0, (2,1) 1, (3,1) 2, (1,2) 3, (2,2) 4, (3,2) 5, (1,3) 6, (2,3) 7, (3,3) 8
+  constexpr int kBlock[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+#pragma unroll
+  for (int b = 0; b < 9; ++b) dst[(size_t)b * MP + idx] = d[kBlock[b]];
+}
+
 // ---- divergence --------------------------------------------------------------
 // i ((a1 s1 + a2 s2) + a3 s3), optionally alpha * g * (.) + beta * dst
 //   (_apply_spectral_divergence!, :120-139)
@@ -451,6 +493,23 @@ gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, h
     hipLaunchKernelGGL((k_gradient<float>), nblocks(M), kOpNT, 0, s, (cpx<float>*)dst,
                        (const cpx<float>*)src, M, nsrc, modes_of(p), tables_of(p));
   return check_launch("gradient");
+}
+
+gbg_status op_symmetric_gradient(gbg_plan* p, void* dst, const void* src, hipStream_t s) {
+  const unsigned M = (unsigned)p->M;
+  if (p->elem == 8)
+    hipLaunchKernelGGL((k_symmetric_gradient<double, 1>), nblocks(M), kOpNT, 0, s,
+                       (ModePack<double, 1>*)dst, (const ModePack<double, 1>*)src, M, modes_of(p),
+                       tables_of(p));
+  else if (M % 2 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0)
+    hipLaunchKernelGGL((k_symmetric_gradient<float, 2>), nblocks(M / 2), kOpNT, 0, s,
+                       (ModePack<float, 2>*)dst, (const ModePack<float, 2>*)src, M / 2,
+                       modes_of(p), tables_of(p));
+  else
+    hipLaunchKernelGGL((k_symmetric_gradient<float, 1>), nblocks(M), kOpNT, 0, s,
+                       (ModePack<float, 1>*)dst, (const ModePack<float, 1>*)src, M, modes_of(p),
+                       tables_of(p));
+  return check_launch("symmetric gradient");
 }
 
 gbg_status op_divergence(gbg_plan* p, void* dst, const void* src, int src_order, int has_band,

@@ -27,7 +27,7 @@ using GBGPeriodicFields: Grid3D, AbstractFieldPhysical, AbstractFieldSpectral, S
     ScalarFieldSpectral, VectorFieldPhysical, VectorFieldSpectral, TensorFieldPhysical,
     TensorFieldSpectral, FieldTransformPlans, FieldSpectralMultipliers, RadialKernel, SpectralKernel,
     field_data, field_grid, field_tensor_order, radial_kernel
-import GBGPeriodicFields: to_spectral!, to_physical!, gradient!, divergence!, curl!, inverse_curl!,
+import GBGPeriodicFields: to_spectral!, to_physical!, gradient!, symmetric_gradient!, divergence!, curl!, inverse_curl!,
     laplacian!, inverse_laplacian!, helmholtz_projection!, convolve!, spectral_band,
     mean_free_projection!, cross!, multiply!, inner, exponential_convolve!, radial_spectrum,
     reality_residual, reality_projection!
@@ -245,6 +245,14 @@ function gradient!(destination::TensorFieldSpectral{2,T}, field::VectorFieldSpec
     p = plan_of(multipliers, field)
     check(ccall((:gbg_gradient, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
                 p.handle, dptr(destination), dptr(field), 1, C_NULL)); destination
+end
+
+function symmetric_gradient!(destination::TensorFieldSpectral{2,T}, field::VectorFieldSpectral{T},
+                             multipliers::GBGMultipliers{T}) where {T<:AbstractFloat}
+    Fields._check_field_domain_compatibility(destination, field)
+    p = plan_of(multipliers, field)
+    check(ccall((:gbg_symmetric_gradient, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                p.handle, dptr(destination), dptr(field), C_NULL)); destination
 end
 
 # A band kernel built on libgbg multipliers: spectral_band(m, lo, hi) and, through

@@ -89,6 +89,11 @@ gbg_status gbg_to_physical(gbg_plan* plan, void* phys, void* spec, int ncomp, vo
 /* gradient! scalar->vector (src_order 0, :812-824, kernel :357-368) and
  * vector->tensor (src_order 1, :827-837, kernel :414-441).  No aliasing. */
 gbg_status gbg_gradient(gbg_plan* plan, void* dst, const void* src, int src_order, void* stream);
+/* symmetric_gradient! vector->symmetric tensor (:1289-1300, kernel
+ * :666-699): S_ii = (i a_i) u_i, S_ij = S_ji = (i/2)(a_j u_i + a_i u_j), the
+ * nine blocks of dst (block i + 3 j) written, each off-diagonal value to both
+ * of its blocks.  No aliasing. */
+gbg_status gbg_symmetric_gradient(gbg_plan* plan, void* dst, const void* src, void* stream);
 /* divergence! vector->scalar (src_order 1) and tensor->vector (src_order 2)
  * with the optional band kernel (has_band: spectral_band(k_lo, k_hi)) and
  * alpha/beta accumulate (:948-1037, _apply_spectral_divergence! :120-139). */

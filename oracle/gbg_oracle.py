@@ -56,14 +56,21 @@ def _workers() -> int:
 @dataclass(frozen=True)
 class Grid3D:
     """Uniform periodic box ``src/Fields.jl:165-200``.  ``n = (Nx, Ny, Nz)``,
-    ``L = (Lx, Ly, Lz)``; the folded (real-FFT) axis is x (``fft_dims[1]``)."""
+    ``L = (Lx, Ly, Lz)``; ``fold`` is the folded (real-FFT, halved) axis,
+    0-based: ``RealFourierLayout(fft_dims).fft_dims[1] - 1`` (``:219-226``),
+    x by default."""
 
     n: tuple
     L: tuple = (TWO_PI, TWO_PI, TWO_PI)
+    fold: int = 0
 
     @property
-    def nh(self) -> int:                       # _rfft_shape, src/Fields.jl:239
-        return self.n[0] // 2 + 1
+    def nh(self) -> int:                       # stored length of the folded axis
+        return self.n[self.fold] // 2 + 1
+
+    @property
+    def extents(self):                         # _rfft_shape, src/Fields.jl:239
+        return tuple(self.nh if a == self.fold else self.n[a] for a in range(3))
 
     @property
     def physical_shape(self):                  # field_physical_shape, :369
@@ -71,7 +78,8 @@ class Grid3D:
 
     @property
     def spectral_shape(self):                  # field_spectral_shape, :388
-        return (self.n[2], self.n[1], self.nh)
+        s = self.extents
+        return (s[2], s[1], s[0])
 
     @property
     def npoints(self) -> int:
@@ -79,7 +87,15 @@ class Grid3D:
 
     @property
     def nmodes(self) -> int:
-        return self.nh * self.n[1] * self.n[2]
+        s = self.extents
+        return s[0] * s[1] * s[2]
+
+    def fold_view(self, a: np.ndarray) -> np.ndarray:
+        """A 1-D array along the folded axis shaped to broadcast over
+        (..., Nz', Ny', Nx')."""
+        shape = [1, 1, 1]
+        shape[2 - self.fold] = a.shape[0]
+        return a.reshape(shape)
 
 
 def axis_wavenumber_values(n: int, L: float, half: bool):
@@ -117,9 +133,9 @@ class Multipliers:
 
     @classmethod
     def build(cls, grid: Grid3D) -> "Multipliers":
-        k1, s1 = axis_wavenumber_values(grid.n[0], grid.L[0], True)
-        k2, s2 = axis_wavenumber_values(grid.n[1], grid.L[1], False)
-        k3, s3 = axis_wavenumber_values(grid.n[2], grid.L[2], False)
+        k1, s1 = axis_wavenumber_values(grid.n[0], grid.L[0], grid.fold == 0)
+        k2, s2 = axis_wavenumber_values(grid.n[1], grid.L[1], grid.fold == 1)
+        k3, s3 = axis_wavenumber_values(grid.n[2], grid.L[2], grid.fold == 2)
         return cls(grid, k1, k2, k3, s1, s2, s3)
 
     # broadcast views onto (Nz, Ny, Nh)
@@ -142,22 +158,33 @@ class Multipliers:
 # Transforms  (src/Fields.jl:1515-1542, 1589-1706)
 # ---------------------------------------------------------------------------
 
-def to_spectral(phys: np.ndarray, workers: int | None = None) -> np.ndarray:
-    """Unnormalised forward rfft over the last three axes (x folded)."""
+def _fft_axes(fold: int):
+    """Array axes of an rfftn over (..., z, y, x) whose last entry (the one
+    rfftn halves) is the folded spatial axis."""
+    axes = [a for a in (-3, -2, -1) if a != -1 - fold]
+    return tuple(axes + [-1 - fold])
+
+
+def to_spectral(phys: np.ndarray, workers: int | None = None, fold: int = 0) -> np.ndarray:
+    """Unnormalised forward rfft over the last three axes, halving the
+    spatial axis ``fold`` (x by default)."""
     w = _workers() if workers is None else workers
+    axes = _fft_axes(fold)
     if _sfft is not None:
-        return _sfft.rfftn(phys, axes=(-3, -2, -1), workers=w)
-    return np.fft.rfftn(phys, axes=(-3, -2, -1))
+        return _sfft.rfftn(phys, axes=axes, workers=w)
+    return np.fft.rfftn(phys, axes=axes)
 
 
 def to_physical(spec: np.ndarray, grid: Grid3D, workers: int | None = None) -> np.ndarray:
-    """Inverse c2r carrying 1/N.  Imaginary parts of the self-conjugate
-    entries of the folded axis are discarded, as FFTW's c2r does."""
+    """Inverse c2r carrying 1/N: full inverses along the other two axes, then
+    c2r along the folded axis, whose self-conjugate entries have their
+    imaginary parts discarded, as FFTW's c2r does."""
     w = _workers() if workers is None else workers
-    s = grid.physical_shape
+    axes = _fft_axes(grid.fold)
+    s = [grid.physical_shape[a] for a in axes]
     if _sfft is not None:
-        return _sfft.irfftn(spec, s=s, axes=(-3, -2, -1), workers=w)
-    return np.fft.irfftn(spec, s=s, axes=(-3, -2, -1))
+        return _sfft.irfftn(spec, s=s, axes=axes, workers=w)
+    return np.fft.irfftn(spec, s=s, axes=axes)
 
 
 def direct_dft_forward(phys: np.ndarray) -> np.ndarray:
@@ -209,6 +236,24 @@ def gradient(s: np.ndarray, m: Multipliers) -> np.ndarray:
     """Scalar -> vector, ``_scalar_gradient_kernel!`` (``:357-368``):
     ``(i*a_j) * s`` with the factor formed first."""
     return np.stack([(1j * m.a1()) * s, (1j * m.a2()) * s, (1j * m.a3()) * s])
+
+
+def symmetric_gradient(u: np.ndarray, m: Multipliers) -> np.ndarray:
+    """Vector -> symmetric tensor, ``_symmetric_gradient_kernel!``
+    (``src/DifferentialOperators.jl:666-699``, entry ``:1289-1300``):
+    ``S_ii = (i*a_i) * u_i`` and ``S_ij = S_ji = (i/2) * (a_j u_i + a_i u_j)``
+    with ``i/2`` the complex ``(0, 1/2)``.  Returned as ``(3, 3, ...)`` with
+    ``[i, j] = S_ij`` (symmetric, so also the ``[j, i]`` view of the stored
+    blocks ``i + 3 j``)."""
+    a = (m.a1(), m.a2(), m.a3())
+    half_i = complex(0.0, 0.5)
+    S = np.empty((3, 3) + u.shape[1:], dtype=np.complex128)
+    for i in range(3):
+        S[i, i] = (1j * a[i]) * u[i]
+        for j in range(i + 1, 3):
+            S[i, j] = half_i * (a[j] * u[i] + a[i] * u[j])
+            S[j, i] = S[i, j]
+    return S
 
 
 def divergence(u: np.ndarray, m: Multipliers) -> np.ndarray:
@@ -286,9 +331,9 @@ def band_mask_float32(grid: Grid3D, k_lo: float, k_hi: float) -> np.ndarray:
         k = unit * freq.astype(T)
         return k * k
 
-    s1 = sq(grid.n[0], grid.L[0], True)
-    s2 = sq(grid.n[1], grid.L[1], False)
-    s3 = sq(grid.n[2], grid.L[2], False)
+    s1 = sq(grid.n[0], grid.L[0], grid.fold == 0)
+    s2 = sq(grid.n[1], grid.L[1], grid.fold == 1)
+    s3 = sq(grid.n[2], grid.L[2], grid.fold == 2)
     q = (s1[None, None, :] + s2[None, :, None]) + s3[:, None, None]
     return (T(k_lo) ** 2 <= q) & (q <= T(k_hi) ** 2)
 
@@ -376,11 +421,13 @@ def multiply(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 def inner_spectral(f: np.ndarray, g: np.ndarray, grid: Grid3D) -> float:
     """Parseval ``inner`` (``src/Reductions.jl:883-925``):
-    ``Re(2*sum f conj(g) - sum_{kx=0} - sum_{kx=Nyq if Nx even}) / N^2``."""
+    ``Re(2*sum f conj(g) - sum_{k_f=0} - sum_{k_f=Nyq if N_f even}) / N^2``
+    over the folded axis f."""
+    ax = -1 - grid.fold
     tot = np.sum(f * np.conj(g))
-    corr = np.sum(f[..., 0] * np.conj(g[..., 0]))
-    if grid.n[0] % 2 == 0:
-        corr = corr + np.sum(f[..., -1] * np.conj(g[..., -1]))
+    corr = np.sum(np.take(f, 0, axis=ax) * np.conj(np.take(g, 0, axis=ax)))
+    if grid.n[grid.fold] % 2 == 0:
+        corr = corr + np.sum(np.take(f, -1, axis=ax) * np.conj(np.take(g, -1, axis=ax)))
     return float(np.real(2.0 * tot - corr)) / float(grid.npoints) ** 2
 
 
@@ -391,12 +438,13 @@ def inner_physical(f: np.ndarray, g: np.ndarray, grid: Grid3D) -> float:
 
 
 def parseval_weight(grid: Grid3D) -> np.ndarray:
-    """``_parseval_weight`` (``src/Reductions.jl:956-965``)."""
+    """``_parseval_weight`` (``src/Reductions.jl:956-965``) along the folded
+    axis, shaped to broadcast over the spectral array."""
     w = np.full(grid.nh, 2.0)
     w[0] = 1.0
-    if grid.n[0] % 2 == 0:
+    if grid.n[grid.fold] % 2 == 0:
         w[-1] = 1.0
-    return w
+    return grid.fold_view(w)
 
 
 # ---------------------------------------------------------------------------
@@ -527,21 +575,27 @@ def radial_spectrum(f: np.ndarray, m: Multipliers, g: np.ndarray | None = None):
 # ---------------------------------------------------------------------------
 
 def _reality_partner(plane: np.ndarray) -> np.ndarray:
-    """The entry at (-kz, -ky) for every (kz, ky) of a (..., Nz, Ny) plane."""
-    nz, ny = plane.shape[-2], plane.shape[-1]
-    jz = (-np.arange(nz)) % nz
-    jy = (-np.arange(ny)) % ny
-    return plane[..., jz, :][..., :, jy]
+    """The entry at (-k_a, -k_b) for every (k_a, k_b) of a (..., Na, Nb) plane."""
+    na, nb = plane.shape[-2], plane.shape[-1]
+    ja = (-np.arange(na)) % na
+    jb = (-np.arange(nb)) % nb
+    return plane[..., ja, :][..., :, jb]
+
+
+def _self_conjugate_planes(grid: Grid3D):
+    """Indices along the folded axis of its self-conjugate planes: 0 and, for
+    an even folded length, the Nyquist plane."""
+    return [0] + ([grid.nh - 1] if grid.n[grid.fold] % 2 == 0 else [])
 
 
 def reality_residual(spec: np.ndarray, grid: Grid3D) -> float:
     """``reality_residual`` (``src/Fields.jl:2858-2869, 2918-2928``): the largest
-    ``|f(k) - conj(f(-k))|`` over the kx = 0 plane and, for even Nx, the Nyquist
-    plane, every component."""
-    planes = [0] + ([grid.nh - 1] if grid.n[0] % 2 == 0 else [])
+    ``|f(k) - conj(f(-k))|`` over the k_f = 0 plane of the folded axis f and,
+    for even N_f, its Nyquist plane, every component."""
+    ax = spec.ndim - 1 - grid.fold
     r = 0.0
-    for p in planes:
-        pl = spec[..., :, :, p]
+    for p in _self_conjugate_planes(grid):
+        pl = np.take(spec, p, axis=ax)
         r = max(r, float(np.max(np.abs(pl - np.conj(_reality_partner(pl))))))
     return r
 
@@ -552,10 +606,12 @@ def reality_projection(spec: np.ndarray, grid: Grid3D) -> np.ndarray:
     ``(f(k) + conj(f(-k))) / 2`` (its partner the conjugate of that, a
     self-paired corner its real part); all other entries are untouched."""
     out = spec.copy()
-    planes = [0] + ([grid.nh - 1] if grid.n[0] % 2 == 0 else [])
-    for p in planes:
-        pl = spec[..., :, :, p]
-        out[..., :, :, p] = (pl + np.conj(_reality_partner(pl))) / 2.0
+    ax = spec.ndim - 1 - grid.fold
+    for p in _self_conjugate_planes(grid):
+        pl = np.take(spec, p, axis=ax)
+        idx = [slice(None)] * spec.ndim
+        idx[ax] = p
+        out[tuple(idx)] = (pl + np.conj(_reality_partner(pl))) / 2.0
     return out
 
 
@@ -567,17 +623,7 @@ def reality_pair_mean(spec: np.ndarray, grid: Grid3D) -> np.ndarray:
     """Project the self-conjugate planes of the folded axis onto the reality
     condition with the pair mean of ``_project_pair!`` (``src/Fields.jl:2949-2958``):
     both members of a pair (k, -k) become (x + conj(y))/2 and its conjugate."""
-    out = spec.copy()
-    Nx, Ny, Nz = grid.n
-    planes = [0] + ([grid.nh - 1] if Nx % 2 == 0 else [])
-    jy = (-np.arange(Ny)) % Ny
-    jz = (-np.arange(Nz)) % Nz
-    for p in planes:
-        pl = out[..., :, :, p]                     # (..., Nz, Ny)
-        partner = pl[..., jz, :][..., :, jy]
-        mean = (pl + np.conj(partner)) / 2.0
-        out[..., :, :, p] = mean
-    return out
+    return reality_projection(spec, grid)
 
 
 def synthetic_field(grid: Grid3D, seed: int, vector: bool = False, m: Multipliers | None = None) -> np.ndarray:

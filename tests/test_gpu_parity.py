@@ -71,6 +71,46 @@ def test_transforms(G, n, dtype):
         assert O.rel_l2(out.download(), O.to_physical(h, g)) < 10 * TOL[dtype], "c2r"
 
 
+CHIRP_GRIDS = [(17, 19, 23), (34, 8, 6), (6, 97, 10), (5, 4, 127), (202, 6, 4), (1009, 4, 3)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", CHIRP_GRIDS)
+def test_chirp_z_axes(G, n, dtype):
+    """Axis lengths with a prime factor above 13 (chirp-z convolution form):
+    forward, both inverse forms and the c2r convention against the oracle on
+    every axis position, operators, the composed u x omega, and a tuned plan."""
+    say(f"[chirp-z] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    tol = TOL[dtype]
+    f = rand_phys(g, 1, 41)
+    ref = O.to_spectral(f)
+    spec = G.to_spectral(G.Field(grid, "physical", 1, dtype).upload(f), plans)
+    got = spec.download()
+    assert O.rel_l2(got, ref) < tol, "forward"
+    assert O.rel_l2(G.to_physical(spec, plans).download(), f) < 10 * tol, "round trip"
+    h = ref + 0.25 * np.conj(ref[..., ::-1, ::-1, :]) * 1j
+    out = G.Field(grid, "physical", 1, dtype)
+    G.to_physical_(out, G.Field(grid, "spectral", 1, dtype).upload(h), plans)
+    assert O.rel_l2(out.download(), O.to_physical(h, g)) < 10 * tol, "c2r"
+    assert O.rel_l2(G.curl(spec, plans).download(), O.curl(ref, m)) < tol
+    u = O.synthetic_field(g, 7, vector=True, m=m)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    kc = O.dealias_cutoff(max(n))          # keeps modes on every axis: a non-vanishing N^
+    dn = G.Field(grid, "spectral", 1, dtype)
+    G.uxw_(dn, du, plans, kc, reductions=False)
+    nref = O.uxw(u, g, kc, m)[0]
+    assert np.linalg.norm(nref) > 1e-3 * np.linalg.norm(u)
+    assert O.rel_l2(dn.download(), nref) < 10 * tol, "u x omega"
+    if n[0] * n[1] * n[2] <= 20000:
+        G.autotune(plans)
+        spec2 = G.to_spectral(G.Field(grid, "physical", 1, dtype).upload(f), plans)
+        assert O.rel_l2(spec2.download(), ref) < tol, "tuned forward"
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("n", [(64, 64, 64), (32, 128, 256), (512, 32, 32), (1024, 16, 16),
                                (16, 1024, 8), (8, 16, 1024)])
@@ -211,6 +251,42 @@ UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), (
              ((1024, 32, 32), 340.0), ((32, 1024, 32), 12.0), ((32, 32, 1024), 12.0),
              # Ny = 512: the y passes of the 512^3 headline, k_uxw_y<T, 512, +-1, ...>
              ((64, 512, 64), None), ((32, 512, 64), 150.0)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,L", [((16, 12, 10), (O.TWO_PI,) * 3), ((8, 7, 5), (1.0, 2.5, 7.0)),
+                                 ((9, 8, 6), (O.TWO_PI,) * 3), ((64, 64, 64), (O.TWO_PI,) * 3),
+                                 ((1, 4, 6), (O.TWO_PI,) * 3)])
+def test_symmetric_gradient(G, n, L, dtype):
+    """symmetric_gradient! against the oracle: even and odd mode counts (FP32
+    takes two modes per 16-byte access only for an even count), Nyquist
+    axes, a non-2pi box; each off-diagonal value written to both blocks, and
+    the diagonal equal to the gradient tensor's."""
+    say(f"[symmetric-gradient] n={n} L={L} dtype={np.dtype(dtype).name}")
+    g = ogrid(n, L)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n, L)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = rand_spec(g, 1, 31)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    S = G.symmetric_gradient(du, plans).download()
+    ref = O.symmetric_gradient(u, m)
+    assert O.rel_l2(S, ref) < TOL[dtype]
+    for i in range(3):
+        for j in range(3):
+            np.testing.assert_array_equal(S[i, j], S[j, i])
+    gt = G.gradient(du, plans).download()
+    for i in range(3):
+        assert O.rel_l2(S[i, i], gt[i, i]) < TOL[dtype]
+    with pytest.raises(G.DimensionMismatch):
+        G.symmetric_gradient_(G.Field(grid, "spectral", 1, dtype), du, plans)
+    with pytest.raises(G.DimensionMismatch):
+        G.symmetric_gradient_(G.Field(grid, "spectral", 2, dtype), G.Field(grid, "spectral", 0, dtype),
+                              plans)
+    L_ = G._lib.load()
+    big = G.Field(grid, "spectral", 2, dtype)
+    st = L_.gbg_symmetric_gradient(plans.handle, big.data, big.data + 16, None)
+    assert st == 4 and b"cannot alias" in L_.gbg_last_error()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -535,8 +611,8 @@ def test_refusals(G):
     say("[refusals]")
     grid = G.Grid3D((8, 8, 8))
     plans = G.FieldTransformPlans(grid)
-    with pytest.raises(G.UnsupportedError):
-        G.FieldTransformPlans(G.Grid3D((17, 8, 8)))
+    with pytest.raises(G.UnsupportedError):     # chirp-z length 4320 > one FP64 LDS pencil
+        G.FieldTransformPlans(G.Grid3D((2053, 8, 8)))
     with pytest.raises(G.ArgumentError):
         G.FieldTransformPlans(grid, np.int32)
     s = G.Field(grid, "spectral", 1)

@@ -99,6 +99,41 @@ def test_curl_of_shear_and_divergence():
     np.testing.assert_allclose(d, ref, atol=1e-12)
 
 
+def test_symmetric_gradient_closed_form_and_nyquist():
+    """Strain rate of u = (sin x + sin 2y, cos 2y + cos 3z, sin x + sin(y+z)) in
+    physical space: S_11 = cos x, S_22 = -2 sin 2y, S_33 = cos(y+z),
+    S_12 = cos 2y, S_13 = cos x / 2, S_23 = (-3 sin 3z + cos(y+z)) / 2.  A
+    Nyquist-only field has zero strain (derivative family zeroed there), and
+    S equals the symmetric part of the pinned vector gradient."""
+    g = _grid16()
+    m = O.Multipliers.build(g)
+    u = np.stack([phys_field(g, lambda x, y, z: np.sin(x) + np.sin(2 * y)),
+                  phys_field(g, lambda x, y, z: np.cos(2 * y) + np.cos(3 * z)),
+                  phys_field(g, lambda x, y, z: np.sin(x) + np.sin(y + z))])
+    S = O.symmetric_gradient(O.to_spectral(u), m)
+    P = np.stack([O.to_physical(S[i], g) for i in range(3)])
+    ref = {(0, 0): lambda x, y, z: np.cos(x),
+           (1, 1): lambda x, y, z: -2 * np.sin(2 * y),
+           (2, 2): lambda x, y, z: np.cos(y + z),
+           (0, 1): lambda x, y, z: np.cos(2 * y),
+           (0, 2): lambda x, y, z: 0.5 * np.cos(x),
+           (1, 2): lambda x, y, z: 0.5 * (-3 * np.sin(3 * z) + np.cos(y + z))}
+    for (i, j), fn in ref.items():
+        np.testing.assert_allclose(P[i, j], phys_field(g, fn), atol=1e-12)
+        np.testing.assert_array_equal(S[i, j], S[j, i])
+    # the symmetric part of the gradient tensor G_ij = (i a_j) u_i
+    uh = O.to_spectral(u)
+    G = np.stack([O.gradient(uh[i], m) for i in range(3)])      # [i, j]
+    np.testing.assert_allclose(S, 0.5 * (G + np.swapaxes(G, 0, 1)), rtol=0, atol=1e-12)
+    # Nyquist modes carry no derivative
+    g8 = O.Grid3D((8, 8, 8))
+    m8 = O.Multipliers.build(g8)
+    w = np.stack([phys_field(g8, lambda x, y, z: np.cos(4 * x)),
+                  phys_field(g8, lambda x, y, z: np.cos(4 * y)),
+                  phys_field(g8, lambda x, y, z: np.cos(4 * z))])
+    assert np.max(np.abs(O.symmetric_gradient(O.to_spectral(w), m8))) < 1e-12
+
+
 def test_laplacian_inverse_and_nyquist_split():
     g = O.Grid3D((8, 8, 8))
     m = O.Multipliers.build(g)
