@@ -29,7 +29,7 @@ from ._lib import (AliasError, ArgumentError, DimensionMismatch, GbgError,  # no
 TWO_PI = 2.0 * math.pi
 
 __all__ = [
-    "Grid3D", "FieldTransformPlans", "Field", "ScalarFieldPhysical", "VectorFieldPhysical",
+    "Grid3D", "RealFourierLayout", "FieldTransformPlans", "Field", "ScalarFieldPhysical", "VectorFieldPhysical",
     "ScalarFieldSpectral", "VectorFieldSpectral", "TensorFieldSpectral", "Stream",
     "to_spectral_", "to_physical_", "to_spectral", "to_physical", "gradient_", "gradient",
     "symmetric_gradient_", "symmetric_gradient",
@@ -92,9 +92,36 @@ class Grid3D:
         Nx, Ny, Nz = self.grid_size
         return (3,) * order + (Nz, Ny, Nx)
 
-    def spectral_shape(self, order=0):
-        Nx, Ny, Nz = self.grid_size
-        return (3,) * order + (Nz, Ny, self.nh)
+    def spectral_extents(self, layout=None):
+        """``_rfft_shape`` (``src/Fields.jl:239``): the folded axis halved."""
+        f = (layout or RealFourierLayout()).fold
+        return tuple(n // 2 + 1 if a == f else n for a, n in enumerate(self.grid_size))
+
+    def spectral_shape(self, order=0, layout=None):
+        sx, sy, sz = self.spectral_extents(layout)
+        return (3,) * order + (sz, sy, sx)
+
+
+@dataclass(frozen=True)
+class RealFourierLayout:
+    """``RealFourierLayout(fft_dims)`` (``src/Fields.jl:219-226``): a
+    permutation of (1, 2, 3) whose first entry is the folded (halved) axis."""
+
+    fft_dims: tuple = (1, 2, 3)
+
+    def __post_init__(self):
+        d = tuple(int(v) for v in self.fft_dims)
+        if len(d) != 3 or sorted(d) != [1, 2, 3]:       # _validate_fft_dims, :122
+            raise ArgumentError(1, f"fft_dims must be a permutation of (1, 2, 3), got {self.fft_dims}")
+        object.__setattr__(self, "fft_dims", d)
+
+    @property
+    def fold(self) -> int:
+        return self.fft_dims[0] - 1
+
+    @property
+    def default(self) -> bool:
+        return self.fft_dims == (1, 2, 3)
 
 
 def dealias_cutoff(n: int, L: float = TWO_PI) -> float:
@@ -108,16 +135,21 @@ class FieldTransformPlans:
     1875-1943``), held by a ``gbg_plan``.  Rank-free: one plan serves scalar,
     vector and tensor fields."""
 
-    def __init__(self, grid: Grid3D, dtype=np.float64):
+    def __init__(self, grid: Grid3D, dtype=np.float64, layout: "RealFourierLayout" = None):
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
             raise ArgumentError(1, "element type must be Float64 or Float32")
         self.grid = grid
         self.dtype = dt
+        self.layout = layout or RealFourierLayout()
         n = (C.c_int * 3)(*[int(v) for v in grid.grid_size])
         L = (C.c_double * 3)(*[float(v) for v in grid.domain_size])
         h = C.c_void_p()
-        check(load().gbg_plan_create(n, L, dt.itemsize, C.byref(h)))
+        if self.layout.default:
+            check(load().gbg_plan_create(n, L, dt.itemsize, C.byref(h)))
+        else:
+            dims = (C.c_int * 3)(*self.layout.fft_dims)
+            check(load().gbg_plan_create_layout(n, L, dt.itemsize, dims, C.byref(h)))
         self.handle = h.value
 
     @property
@@ -155,7 +187,8 @@ class Field:
     tensor order R (0, 1, 2), element type and grid.  ``data`` is the device
     pointer of the contiguous column-major storage."""
 
-    def __init__(self, grid: Grid3D, representation: str, order: int, dtype=np.float64):
+    def __init__(self, grid: Grid3D, representation: str, order: int, dtype=np.float64,
+                 layout: "RealFourierLayout" = None):
         if representation not in ("physical", "spectral"):
             raise ArgumentError(1, "representation must be physical or spectral")
         if order not in (0, 1, 2):
@@ -164,8 +197,9 @@ class Field:
         self.representation = representation
         self.order = order
         self.dtype = np.dtype(dtype)
+        self.layout = layout or RealFourierLayout()
         if self.spectral:
-            self.shape = grid.spectral_shape(order)
+            self.shape = grid.spectral_shape(order, self.layout)
             self.npdtype = np.dtype(np.complex128 if self.dtype == np.float64 else np.complex64)
         else:
             self.shape = grid.physical_shape(order)
@@ -208,7 +242,7 @@ class Field:
 
     def similar(self, representation=None, order=None) -> "Field":
         return Field(self.grid, representation or self.representation,
-                     self.order if order is None else order, self.dtype)
+                     self.order if order is None else order, self.dtype, self.layout)
 
     def free(self):
         if getattr(self, "data", None):
@@ -222,24 +256,24 @@ class Field:
             pass
 
 
-def ScalarFieldPhysical(grid, dtype=np.float64):
-    return Field(grid, "physical", 0, dtype)
+def ScalarFieldPhysical(grid, dtype=np.float64, layout=None):
+    return Field(grid, "physical", 0, dtype, layout)
 
 
-def VectorFieldPhysical(grid, dtype=np.float64):
-    return Field(grid, "physical", 1, dtype)
+def VectorFieldPhysical(grid, dtype=np.float64, layout=None):
+    return Field(grid, "physical", 1, dtype, layout)
 
 
-def ScalarFieldSpectral(grid, dtype=np.float64):
-    return Field(grid, "spectral", 0, dtype)
+def ScalarFieldSpectral(grid, dtype=np.float64, layout=None):
+    return Field(grid, "spectral", 0, dtype, layout)
 
 
-def VectorFieldSpectral(grid, dtype=np.float64):
-    return Field(grid, "spectral", 1, dtype)
+def VectorFieldSpectral(grid, dtype=np.float64, layout=None):
+    return Field(grid, "spectral", 1, dtype, layout)
 
 
-def TensorFieldSpectral(grid, dtype=np.float64):
-    return Field(grid, "spectral", 2, dtype)
+def TensorFieldSpectral(grid, dtype=np.float64, layout=None):
+    return Field(grid, "spectral", 2, dtype, layout)
 
 
 # ---- argument checks mirroring the reference's refusals ------------------------
@@ -250,6 +284,8 @@ def _same_domain(a: Field, b: Field):
         raise ArgumentError(1, "fields live on different grids")
     if a.dtype != b.dtype:
         raise ArgumentError(1, "field scalar types must be equal")
+    if a.layout != b.layout:
+        raise ArgumentError(1, "fields were built for different Fourier layouts")
 
 
 def _with_plan(f: Field, plans: FieldTransformPlans):
@@ -258,6 +294,14 @@ def _with_plan(f: Field, plans: FieldTransformPlans):
         raise ArgumentError(1, "field grid does not match the plans' grid")
     if f.dtype != plans.dtype:
         raise ArgumentError(1, "field and plans scalar types must be equal")
+    if f.layout != plans.layout:
+        raise ArgumentError(1, "field Fourier layout must be equal to FieldTransformPlans Fourier layout")
+
+
+def _default_layout(f: Field, what: str):
+    """The batch and the server make plans of the default layout."""
+    if not f.layout.default:
+        raise UnsupportedError(3, f"{what} takes fields of the default layout RealFourierLayout((1, 2, 3))")
 
 
 def _need(f: Field, rep: str, order=None):
@@ -304,13 +348,13 @@ def to_physical_(dst: Field, src: Field, plans: FieldTransformPlans, scratch: Fi
 
 
 def to_spectral(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
-    out = Field(src.grid, "spectral", src.order, src.dtype)
+    out = Field(src.grid, "spectral", src.order, src.dtype, src.layout)
     return to_spectral_(out, src, plans, stream)
 
 
 def to_physical(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
     """Allocating form: copies the source first so it is preserved."""
-    out = Field(src.grid, "physical", src.order, src.dtype)
+    out = Field(src.grid, "physical", src.order, src.dtype, src.layout)
     tmp = src.similar()
     return to_physical_(out, src, plans, scratch=tmp, stream=stream)
 
@@ -330,7 +374,8 @@ def gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -
 
 
 def gradient(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
-    return gradient_(Field(src.grid, "spectral", src.order + 1, src.dtype), src, plans, stream)
+    return gradient_(Field(src.grid, "spectral", src.order + 1, src.dtype, src.layout), src, plans,
+                     stream)
 
 
 def symmetric_gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stream=None) -> Field:
@@ -344,7 +389,8 @@ def symmetric_gradient_(dst: Field, src: Field, plans: FieldTransformPlans, stre
 
 
 def symmetric_gradient(src: Field, plans: FieldTransformPlans, stream=None) -> Field:
-    return symmetric_gradient_(Field(src.grid, "spectral", 2, src.dtype), src, plans, stream)
+    return symmetric_gradient_(Field(src.grid, "spectral", 2, src.dtype, src.layout), src, plans,
+                               stream)
 
 
 def divergence_(dst: Field, src: Field, plans: FieldTransformPlans, kernel=None, alpha=1.0,
@@ -658,6 +704,7 @@ class Batch:
         self.keep = []
 
     def _add(self, kind, f: Field, kc):
+        _default_layout(f, "the batch")
         g = f.grid
         if kc is None:
             kc = min(dealias_cutoff(n, L) for n, L in zip(g.grid_size, g.domain_size))
@@ -724,6 +771,7 @@ class Server:
             raise ArgumentError(1, f"unknown job kind {kind}")
         rin, rout, order = self._IO[kind]
         _need(in0, rin, order)
+        _default_layout(in0, "the server")
         _need(out, rout, in0.order)
         _same_domain(out, in0)
         if kind == self.DEALIASED_PRODUCT:

@@ -49,6 +49,8 @@ SIGNATURES = {
     "gbg_stream_destroy": (st, [vp]),
     "gbg_stream_sync": (st, [vp]),
     "gbg_plan_create": (st, [C.POINTER(C.c_int), C.POINTER(C.c_double), i32, C.POINTER(vp)]),
+    "gbg_plan_create_layout": (st, [C.POINTER(C.c_int), C.POINTER(C.c_double), i32,
+                                    C.POINTER(C.c_int), C.POINTER(vp)]),
     "gbg_plan_destroy": (st, [vp]),
     "gbg_plan_query": (st, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
     "gbg_to_spectral": (st, [vp, vp, vp, i32, vp]),

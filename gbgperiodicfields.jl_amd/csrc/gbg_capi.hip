@@ -211,19 +211,37 @@ static void destroy_plan(gbg_plan* p) {
   delete p;
 }
 
-static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_plan** out) {
-  if (!n || !L || !out) return fail(GBG_E_ARG, "null argument");
+static gbg_status create_plan(const int n[3], const double L[3], int elem, const int fft_dims[3],
+                              gbg_plan** out) {
+  if (!n || !L || !out || !fft_dims) return fail(GBG_E_ARG, "null argument");
   *out = nullptr;
   if (elem != 4 && elem != 8) return fail(GBG_E_ARG, "elem_bytes must be 4 (Float32) or 8 (Float64)");
   for (int a = 0; a < 3; ++a) {
     if (n[a] < 1) return fail(GBG_E_ARG, "grid sizes must be positive");
     if (!(L[a] > 0.0) || !std::isfinite(L[a])) return fail(GBG_E_ARG, "domain lengths must be positive and finite");
   }
-  const size_t M = (size_t)(n[0] / 2 + 1) * n[1] * n[2];
+  // _validate_fft_dims (src/Fields.jl:122)
+  {
+    bool seen[4] = {false, false, false, false};
+    for (int a = 0; a < 3; ++a) {
+      const int d = fft_dims[a];
+      if (d < 1 || d > 3 || seen[d]) return fail(GBG_E_ARG, "fft_dims must be a permutation of (1, 2, 3)");
+      seen[d] = true;
+    }
+  }
+  const int fold = fft_dims[0] - 1;
+  int ext[3];
+  for (int a = 0; a < 3; ++a) ext[a] = a == fold ? n[a] / 2 + 1 : n[a];
+  const size_t M = (size_t)ext[0] * ext[1] * ext[2];
   if (3 * M >= (size_t)1 << 32) return fail(GBG_E_UNSUPPORTED, "grid too large for 32-bit mode indexing");
   gbg_plan* p = new (std::nothrow) gbg_plan();
   if (!p) return fail(GBG_E_OOM, "host allocation");
   std::memset(p, 0, sizeof(*p));
+  p->fold = fold;
+  for (int a = 0; a < 3; ++a) {
+    p->fft_dims[a] = fft_dims[a];
+    p->s[a] = ext[a];
+  }
   for (int a = 0; a < 3; ++a) {
     p->n[a] = n[a];
     p->L[a] = L[a];
@@ -258,7 +276,7 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
     p->use_fast[a] = 1;
     p->bsel[a] = 0;
   }
-  p->nh = n[0] / 2 + 1;
+  p->nh = ext[0];
   p->M = M;
   p->P = (size_t)n[0] * n[1] * n[2];
   gbg_status st = check_hip(hipGetDevice(&p->device), "hipGetDevice");
@@ -271,7 +289,7 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, gbg_p
   }
   for (int a = 0; a < 3; ++a) {
     std::vector<double> kd, ks;
-    axis_tables(n[a], L[a], a == 0, kd, ks);
+    axis_tables(n[a], L[a], a == fold, kd, ks);
     const size_t bytes = sizeof(double) * kd.size();
     if ((st = check_hip(hipMalloc((void**)&p->kd[a], bytes), "wavenumber table")) ||
         (st = check_hip(hipMalloc((void**)&p->ks[a], bytes), "wavenumber table")) ||
@@ -455,7 +473,15 @@ gbg_status gbg_stream_sync(void* stream) {
 
 gbg_status gbg_plan_create(const int n[3], const double L[3], int elem_bytes, gbg_plan** out) {
   GBG_GUARD_BEGIN
-  return create_plan(n, L, elem_bytes, out);
+  const int dims[3] = {1, 2, 3};
+  return create_plan(n, L, elem_bytes, dims, out);
+  GBG_GUARD_END
+}
+
+gbg_status gbg_plan_create_layout(const int n[3], const double L[3], int elem_bytes,
+                                  const int fft_dims[3], gbg_plan** out) {
+  GBG_GUARD_BEGIN
+  return create_plan(n, L, elem_bytes, fft_dims, out);
   GBG_GUARD_END
 }
 

@@ -114,6 +114,83 @@ __global__ __launch_bounds__(kNT) void k_c2r_x(const cpx<T>* __restrict__ in, T*
   }
 }
 
+// R2C along a strided folded axis (y or z of a permuted layout): pencil pairs
+// (2q, 2q + 1) of the pencil enumeration are transformed as z = a + i b, then
+// split into the two half spectra as in k_r2c_x.  `pi` addresses the real
+// input, `po` the complex output (same pencil enumeration, their own strides).
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_r2c_strided(const T* __restrict__ in, cpx<T>* __restrict__ out,
+                                                     Strided pi, Strided po, int n, int nh, int B,
+                                                     int ld, AxisPlan ap,
+                                                     const cpx<T>* __restrict__ tw) {
+  __shared__ cpx<T> s[2 * Lds<T>::kElems];
+  const long long pair0 = fast::xcd_tile() * B;
+  const long long ci = (long long)blockIdx.y * pi.cstride, co = (long long)blockIdx.y * po.cstride;
+  auto off = [](const Strided& d, long long p) { return (p / d.inner) * d.ostride + (p % d.inner); };
+  for (int idx = threadIdx.x; idx < B * n; idx += kNT) {
+    const int q = idx / n, e = idx - q * n;
+    const long long p0 = 2 * (pair0 + q), p1 = p0 + 1;
+    const T a = p0 < pi.npencil ? in[ci + off(pi, p0) + e * pi.se] : T(0);
+    const T b = p1 < pi.npencil ? in[ci + off(pi, p1) + e * pi.se] : T(0);
+    s[q * ld + e] = mk<T>(a, b);
+  }
+  __syncthreads();
+  const cpx<T>* rs = fft_any<T, -1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  for (int idx = threadIdx.x; idx < B * nh; idx += kNT) {
+    const int q = idx / nh, k = idx - q * nh;
+    const long long p0 = 2 * (pair0 + q), p1 = p0 + 1;
+    if (p0 >= po.npencil) continue;
+    const cpx<T> z = rs[q * ld + k];
+    const cpx<T> zc = rs[q * ld + (k == 0 ? 0 : n - k)];
+    out[co + off(po, p0) + k * po.se] = mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y));
+    if (p1 < po.npencil)
+      out[co + off(po, p1) + k * po.se] = mk<T>(T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x));
+  }
+}
+
+// C2R along a strided folded axis: the Hermitian completion of two pencils'
+// half spectra (self-conjugate entries taken real) as Z = A + i B, inverse
+// transform, real part to the first pencil and imaginary part to the second,
+// scaled by `scale`.
+template <typename T>
+__global__ __launch_bounds__(kNT) void k_c2r_strided(const cpx<T>* __restrict__ in, T* __restrict__ out,
+                                                     Strided pi, Strided po, int n, int nh, int B,
+                                                     int ld, AxisPlan ap,
+                                                     const cpx<T>* __restrict__ tw, T scale) {
+  __shared__ cpx<T> s[2 * Lds<T>::kElems];
+  const long long pair0 = fast::xcd_tile() * B;
+  const long long ci = (long long)blockIdx.y * pi.cstride, co = (long long)blockIdx.y * po.cstride;
+  auto off = [](const Strided& d, long long p) { return (p / d.inner) * d.ostride + (p % d.inner); };
+  const int nyq = (n % 2 == 0) ? n / 2 : -1;
+  for (int idx = threadIdx.x; idx < B * n; idx += kNT) {
+    const int q = idx / n, k = idx - q * n;
+    const long long p0 = 2 * (pair0 + q), p1 = p0 + 1;
+    const bool upper = k >= nh;
+    const int kk = upper ? n - k : k;
+    cpx<T> a = p0 < pi.npencil ? in[ci + off(pi, p0) + kk * pi.se] : mk<T>(0, 0);
+    cpx<T> b = p1 < pi.npencil ? in[ci + off(pi, p1) + kk * pi.se] : mk<T>(0, 0);
+    if (kk == 0 || kk == nyq) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    if (upper) {
+      a.y = -a.y;
+      b.y = -b.y;
+    }
+    s[q * ld + k] = mk<T>(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  const cpx<T>* rs = fft_any<T, 1>(s, s + Lds<T>::kElems, ld, ap, B, tw);
+  for (int idx = threadIdx.x; idx < B * n; idx += kNT) {
+    const int q = idx / n, e = idx - q * n;
+    const long long p0 = 2 * (pair0 + q), p1 = p0 + 1;
+    if (p0 >= po.npencil) continue;
+    const cpx<T> z = rs[q * ld + e];
+    out[co + off(po, p0) + e * po.se] = z.x * scale;
+    if (p1 < po.npencil) out[co + off(po, p1) + e * po.se] = z.y * scale;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // register-resident passes for power-of-two lengths (fft_fast.hpp)
 // ----------------------------------------------------------------------------
@@ -355,25 +432,40 @@ static int ilog2(int b) {
   return s;
 }
 
+// pencils along `axis` of a column-major array of extents e[3]: pencil p,
+// element k at (p / inner) ostride + (p mod inner) + k se
+static Strided pencils(const long long e[3], int axis, long long cstride) {
+  Strided d;
+  if (axis == 0) {        // x: pencils over (y, z)
+    d.npencil = e[1] * e[2];
+    d.inner = 1;
+    d.ostride = e[0];
+    d.se = 1;
+  } else if (axis == 1) { // y: pencils over (z, x)
+    d.npencil = e[2] * e[0];
+    d.inner = e[0];
+    d.ostride = e[0] * e[1];
+    d.se = e[0];
+  } else {                // z: pencils over (y, x)
+    d.npencil = e[1] * e[0];
+    d.inner = e[1] * e[0];
+    d.ostride = 0;
+    d.se = e[1] * e[0];
+  }
+  d.cstride = cstride;
+  d.B = 1;
+  d.bshift = 0;
+  d.ld = 0;
+  return d;
+}
+
 template <typename T>
 static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* in, cpx<T>* out,
                                  int ncomp, T scale, hipStream_t s) {
   const AxisPlan& ap = p->ax[axis];
   if (ap.n == 1 && in == out) return GBG_OK;
-  Strided d;
-  const long long nh = p->nh, ny = p->n[1];
-  if (axis == 1) {        // y: pencils over (z, kx)
-    d.npencil = (long long)p->n[2] * nh;
-    d.inner = nh;
-    d.ostride = nh * ny;
-    d.se = nh;
-  } else {                // z: pencils over (ky, kx)
-    d.npencil = ny * nh;
-    d.inner = ny * nh;
-    d.ostride = 0;
-    d.se = ny * nh;
-  }
-  d.cstride = (long long)p->M;
+  const long long ext[3] = {p->s[0], p->s[1], p->s[2]};
+  Strided d = pencils(ext, axis, (long long)p->M);
   d.B = p->bsel[axis] ? p->bsel[axis] : pick_b(pencil_len(ap), Lds<T>::kCap);
   d.bshift = ilog2(d.B);
   d.ld = pencil_len(ap) + 1;
@@ -389,8 +481,58 @@ static gbg_status launch_strided(gbg_plan* p, int axis, int dir, const cpx<T>* i
   return check_launch("c2c pass");
 }
 
+// the real <-> half-spectrum pass along a strided folded axis (y or z)
+template <typename T>
+static gbg_status launch_fold_pass(gbg_plan* p, int dir, const void* in, void* out, int ncomp,
+                                   T scale, hipStream_t s) {
+  const int f = p->fold;
+  const AxisPlan& ap = p->ax[f];
+  const long long pe[3] = {p->n[0], p->n[1], p->n[2]};
+  const long long se[3] = {p->s[0], p->s[1], p->s[2]};
+  const Strided dp = pencils(pe, f, (long long)p->P), ds = pencils(se, f, (long long)p->M);
+  const int len = pencil_len(ap);
+  const int B = p->bsel[f] ? p->bsel[f] : pick_b(len, Lds<T>::kCap);
+  const long long pairs = (dp.npencil + 1) / 2;
+  dim3 grid(fast::xcd_grid((pairs + B - 1) / B), (unsigned)ncomp);
+  const cpx<T>* tw = reinterpret_cast<const cpx<T>*>(p->tw[f]);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_r2c_strided<T>), grid, dim3(kNT), 0, s, (const T*)in, (cpx<T>*)out, dp, ds,
+                       p->n[f], p->s[f], B, len + 1, ap, tw);
+  else
+    hipLaunchKernelGGL((k_c2r_strided<T>), grid, dim3(kNT), 0, s, (const cpx<T>*)in, (T*)out, ds, dp,
+                       p->n[f], p->s[f], B, len + 1, ap, tw, scale);
+  return check_launch(dir < 0 ? "r2c folded pass" : "c2r folded pass");
+}
+
+// a permuted layout: R2C along the folded axis, then C2C along the other two
+template <typename T>
+static gbg_status fwd_folded(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, hipStream_t s) {
+  gbg_status st = launch_fold_pass<T>(p, -1, phys, spec, ncomp, T(1), s);
+  for (int a = 0; a < 3 && !st; ++a)
+    if (a != p->fold) st = launch_strided<T>(p, a, -1, spec, spec, ncomp, T(1), s);
+  return st;
+}
+
+// its inverse: C2C along the two full axes (the first out of place into the
+// work buffer), then C2R along the folded axis carrying 1/N
+template <typename T>
+static gbg_status inv_folded(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, int ncomp,
+                             hipStream_t s) {
+  gbg_status st = GBG_OK;
+  bool first = true;
+  for (int a = 2; a >= 0 && !st; --a) {
+    if (a == p->fold) continue;
+    st = launch_strided<T>(p, a, 1, first ? spec_in : work, work, ncomp, T(1), s);
+    first = false;
+  }
+  if (st) return st;
+  const T scale = T(1.0 / ((double)p->n[0] * (double)p->n[1] * (double)p->n[2]));
+  return launch_fold_pass<T>(p, 1, work, phys, ncomp, scale, s);
+}
+
 template <typename T>
 static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, hipStream_t s) {
+  if (p->fold != 0) return fwd_folded<T>(p, spec, phys, ncomp, s);
   const int nx = p->n[0], nh = p->nh;
   const long long nrows = (long long)ncomp * p->n[1] * p->n[2];
   const cpx<T>* twx = reinterpret_cast<const cpx<T>*>(p->tw[0]);
@@ -413,6 +555,7 @@ static gbg_status fwd_impl(gbg_plan* p, cpx<T>* spec, const T* phys, int ncomp, 
 template <typename T>
 static gbg_status inv_impl(gbg_plan* p, T* phys, cpx<T>* spec_in, cpx<T>* work, int ncomp,
                            hipStream_t s) {
+  if (p->fold != 0) return inv_folded<T>(p, phys, spec_in, work, ncomp, s);
   gbg_status st;
   if ((st = launch_strided<T>(p, 2, 1, spec_in, work, ncomp, T(1), s))) return st;
   if ((st = launch_strided<T>(p, 1, 1, work, work, ncomp, T(1), s))) return st;

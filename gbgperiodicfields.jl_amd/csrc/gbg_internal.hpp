@@ -108,7 +108,12 @@ struct gbg_plan {
   int n[3];          // Nx, Ny, Nz
   double L[3];       // box lengths
   int elem;          // 4 or 8
-  int nh;            // Nx/2 + 1
+  // the folded (real-FFT, halved) axis, 0 = x unless the plan was made for a
+  // permuted RealFourierLayout, and the layout's transform order fft_dims
+  int fold;
+  int fft_dims[3];
+  int s[3];          // spectral extents: n, with n[fold] / 2 + 1 on the folded axis
+  int nh;            // s[0], the spectral x extent (Nx/2 + 1 when x is folded)
   size_t M;          // stored modes per component
   size_t P;          // grid points per component
   int device;

@@ -90,7 +90,9 @@ __global__ __launch_bounds__(kNT) void k_exponential_combine(cpx<T>* dst, const 
 // (_spectral_shell_bin_kernel!, src/Reductions.jl:1309-1327).  Deterministic FP64 sums
 // instead of the reference's atomics: along an x row |k| grows with kx, so a wave reads 64
 // consecutive kx of a row, adds up equal-shell runs with a segmented scan, and the run ends
-// add into the wave's own LDS histogram (distinct shells, rows in a fixed order).  The
+// add into the wave's own LDS histogram (distinct shells, rows in a fixed order).  When y
+// or z is the folded axis the x row holds every frequency; lane j then reads kx index
+// 0, 1, n-1, 2, n-2, ... (frequencies 0, 1, -1, 2, -2, ...), along which |k| grows too.  The
 // workgroup writes the sum of its four histograms to part[shell][workgroup], a second
 // kernel adds each shell's row in a fixed order.
 constexpr int kRadialBlocks = 4096;
@@ -100,8 +102,8 @@ template <typename T>
 __global__ __launch_bounds__(kNT) void k_radial_bin(const cpx<T>* __restrict__ f,
                                                     const cpx<T>* __restrict__ g, size_t M,
                                                     int ncomp, unsigned nh, unsigned ny,
-                                                    unsigned nrows, int nyq, Tables t, double dk,
-                                                    int nsh, double* __restrict__ part) {
+                                                    unsigned nrows, int fold, int nyq, Tables t,
+                                                    double dk, int nsh, double* __restrict__ part) {
   extern __shared__ double hist[];   // [4][nsh]
   for (int i = threadIdx.x; i < 4 * nsh; i += kNT) hist[i] = 0.0;
   __syncthreads();
@@ -111,10 +113,14 @@ __global__ __launch_bounds__(kNT) void k_radial_bin(const cpx<T>* __restrict__ f
   for (unsigned r = blockIdx.x * 4 + w; r < nrows; r += nw) {
     const unsigned ky = r % ny, kz = r / ny;
     const double s2 = t.ks2[ky], s3 = t.ks3[kz];
+    // the Parseval weight of a row when y or z is the folded axis
+    const int kf = fold == 1 ? (int)ky : (int)kz;
+    const double wrow = (kf == 0 || kf == nyq) ? 1.0 : 2.0;
     const size_t base = (size_t)r * nh;
     for (unsigned c0 = 0; c0 < nh; c0 += 64) {
-      const unsigned kx = c0 + lane;
-      const bool valid = kx < nh;
+      const unsigned j = c0 + lane;
+      const unsigned kx = fold == 0 || j == 0 ? j : ((j & 1u) ? (j + 1) / 2 : nh - j / 2);
+      const bool valid = j < nh;
       int sh = -1;
       double v = 0.0;
       if (valid) {
@@ -130,7 +136,7 @@ __global__ __launch_bounds__(kNT) void k_radial_bin(const cpx<T>* __restrict__ f
             acc += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
           }
         }
-        v = ((kx == 0 || (int)kx == nyq) ? 1.0 : 2.0) * acc;
+        v = (fold == 0 ? (((int)kx == 0 || (int)kx == nyq) ? 1.0 : 2.0) : wrow) * acc;
       }
       // segmented inclusive scan over runs of equal shell (runs are contiguous)
 #pragma unroll
@@ -166,29 +172,37 @@ __global__ __launch_bounds__(kNT) void k_radial_final(const double* __restrict__
 }
 
 // ---- reality condition ---------------------------------------------------------------
-// Planes: kx = 0 and, for even Nx, kx = nh - 1.  Within a plane the entry (ky, kz) pairs
-// with ((-ky) mod Ny, (-kz) mod Nz).  Work item = (ky, kz, component, plane) as in
-// _reality_projection_kernel! (src/Fields.jl:2974-2990); the owner of a pair (ky < ky', or
-// equal and kz <= kz') reads both members and writes both (_project_pair!, :2949-2958).
+// Planes of the folded axis f: k_f = 0 and, for even N_f, its Nyquist plane.  Within a
+// plane, spanned by the other two axes a and b, the entry (j, l) pairs with
+// ((-j) mod na, (-l) mod nb).  Work item = (j, l, component, plane) as in
+// _reality_projection_kernel! (src/Fields.jl:2974-2990); the owner of a pair (j < j', or
+// equal and l <= l') reads both members and writes both (_project_pair!, :2949-2958).
+// Strides are in entries of one component (x folded: a = y, b = z, sf = 1).
+struct Planes {
+  unsigned na, nb;
+  size_t sa, sb, sf;
+  unsigned last;   // index of the second plane along the folded axis
+  int nplanes;
+};
+
 template <typename T>
-__global__ __launch_bounds__(kNT) void k_reality_project(cpx<T>* data, size_t M, unsigned nh,
-                                                         unsigned ny, unsigned nz, int ncomp,
-                                                         int nplanes) {
+__global__ __launch_bounds__(kNT) void k_reality_project(cpx<T>* data, size_t M, Planes pl,
+                                                         int ncomp) {
   const size_t item = (size_t)blockIdx.x * kNT + threadIdx.x;
-  const size_t per_plane = (size_t)ny * nz * ncomp;
-  if (item >= per_plane * nplanes) return;
+  const size_t per_plane = (size_t)pl.na * pl.nb * ncomp;
+  if (item >= per_plane * pl.nplanes) return;
   size_t rest = item;
-  const unsigned j = rest % ny;
-  rest /= ny;
-  const unsigned l = rest % nz;
-  rest /= nz;
+  const unsigned j = rest % pl.na;
+  rest /= pl.na;
+  const unsigned l = rest % pl.nb;
+  rest /= pl.nb;
   const unsigned c = rest % ncomp;
-  const unsigned p = (rest / ncomp) == 0 ? 0 : nh - 1;
-  const unsigned jp = (ny - j) % ny, lp = (nz - l) % nz;
+  const size_t p = (rest / ncomp) == 0 ? 0 : (size_t)pl.last * pl.sf;
+  const unsigned jp = (pl.na - j) % pl.na, lp = (pl.nb - l) % pl.nb;
   if (!(j < jp || (j == jp && l <= lp))) return;
-  const size_t off = (size_t)c * M;
-  const size_t ix = off + p + (size_t)nh * (j + (size_t)ny * l);
-  const size_t iy = off + p + (size_t)nh * (jp + (size_t)ny * lp);
+  const size_t off = (size_t)c * M + p;
+  const size_t ix = off + pl.sa * j + pl.sb * l;
+  const size_t iy = off + pl.sa * jp + pl.sb * lp;
   const cpx<T> x = data[ix];
   const cpx<T> y = ix == iy ? x : data[iy];
   const cpx<T> m = mk<T>((x.x + y.x) / T(2), (x.y - y.y) / T(2));
@@ -199,25 +213,24 @@ __global__ __launch_bounds__(kNT) void k_reality_project(cpx<T>* data, size_t M,
 // largest |f(k) - conj f(-k)| over the planes (reality_residual, src/Fields.jl:2858-2869)
 template <typename T>
 __global__ __launch_bounds__(kNT) void k_reality_residual(const cpx<T>* __restrict__ data, size_t M,
-                                                          unsigned nh, unsigned ny, unsigned nz,
-                                                          int ncomp, int nplanes,
+                                                          Planes pl, int ncomp,
                                                           double* __restrict__ part) {
   __shared__ double red[kNT];
-  const size_t total = (size_t)ny * nz * ncomp * nplanes;
+  const size_t total = (size_t)pl.na * pl.nb * ncomp * pl.nplanes;
   double mx = 0.0;
   for (size_t item = (size_t)blockIdx.x * kNT + threadIdx.x; item < total;
        item += (size_t)gridDim.x * kNT) {
     size_t rest = item;
-    const unsigned j = rest % ny;
-    rest /= ny;
-    const unsigned l = rest % nz;
-    rest /= nz;
+    const unsigned j = rest % pl.na;
+    rest /= pl.na;
+    const unsigned l = rest % pl.nb;
+    rest /= pl.nb;
     const unsigned c = rest % ncomp;
-    const unsigned p = (rest / ncomp) == 0 ? 0 : nh - 1;
-    const unsigned jp = (ny - j) % ny, lp = (nz - l) % nz;
-    const size_t off = (size_t)c * M;
-    const cpx<T> x = data[off + p + (size_t)nh * (j + (size_t)ny * l)];
-    const cpx<T> y = data[off + p + (size_t)nh * (jp + (size_t)ny * lp)];
+    const size_t p = (rest / ncomp) == 0 ? 0 : (size_t)pl.last * pl.sf;
+    const unsigned jp = (pl.na - j) % pl.na, lp = (pl.nb - l) % pl.nb;
+    const size_t off = (size_t)c * M + p;
+    const cpx<T> x = data[off + pl.sa * j + pl.sb * l];
+    const cpx<T> y = data[off + pl.sa * jp + pl.sb * lp];
     const T dx = x.x - y.x, dy = x.y + y.y;   // f(k) - conj f(-k), in T
     const double a = sqrt((double)dx * (double)dx + (double)dy * (double)dy);
     mx = a > mx ? a : mx;
@@ -252,11 +265,11 @@ gbg_status op_exponential(gbg_plan* p, void* dst, const void* src, int ncomp, do
   const ExpArgs e{coefficient, offset_scale, alpha, beta};
   if (p->elem == 8)
     hipLaunchKernelGGL((k_exponential<double>), grid_of(M), kNT, 0, s, (cpx<double>*)dst,
-                       (const cpx<double>*)src, M, ncomp, (unsigned)p->nh, (unsigned)p->n[1],
+                       (const cpx<double>*)src, M, ncomp, (unsigned)p->s[0], (unsigned)p->s[1],
                        tables_of(p), (const cpx<double>*)offset, e);
   else
     hipLaunchKernelGGL((k_exponential<float>), grid_of(M), kNT, 0, s, (cpx<float>*)dst,
-                       (const cpx<float>*)src, M, ncomp, (unsigned)p->nh, (unsigned)p->n[1],
+                       (const cpx<float>*)src, M, ncomp, (unsigned)p->s[0], (unsigned)p->s[1],
                        tables_of(p), (const cpx<float>*)offset, e);
   return check_launch("exponential convolve");
 }
@@ -268,12 +281,12 @@ gbg_status op_exponential_combine(gbg_plan* p, void* dst, double alpha, const vo
   const ExpArgs e{coefficient, offset_scale, alpha, beta};
   if (p->elem == 8)
     hipLaunchKernelGGL((k_exponential_combine<double>), grid_of(M), kNT, 0, s, (cpx<double>*)dst,
-                       (const cpx<double>*)a, (const cpx<double>*)b, M, ncomp, (unsigned)p->nh,
-                       (unsigned)p->n[1], tables_of(p), (const cpx<double>*)offset, e);
+                       (const cpx<double>*)a, (const cpx<double>*)b, M, ncomp, (unsigned)p->s[0],
+                       (unsigned)p->s[1], tables_of(p), (const cpx<double>*)offset, e);
   else
     hipLaunchKernelGGL((k_exponential_combine<float>), grid_of(M), kNT, 0, s, (cpx<float>*)dst,
-                       (const cpx<float>*)a, (const cpx<float>*)b, M, ncomp, (unsigned)p->nh,
-                       (unsigned)p->n[1], tables_of(p), (const cpx<float>*)offset, e);
+                       (const cpx<float>*)a, (const cpx<float>*)b, M, ncomp, (unsigned)p->s[0],
+                       (unsigned)p->s[1], tables_of(p), (const cpx<float>*)offset, e);
   return check_launch("exponential combine");
 }
 
@@ -302,18 +315,18 @@ gbg_status op_radial_spectrum(gbg_plan* p, const void* f, const void* g, int nco
   if (nsh != n) return fail(GBG_E_DIM, "nshells does not match the plan's shell count");
   if (nsh > kRadialMaxShells)
     return fail(GBG_E_UNSUPPORTED, "more than 2048 spectral shells (grid too anisotropic)");
-  const unsigned nrows = (unsigned)(p->n[1] * p->n[2]);
+  const unsigned nrows = (unsigned)(p->s[1] * p->s[2]);
   const unsigned nb = nrows / 4 + 1 < (unsigned)kRadialBlocks ? nrows / 4 + 1 : kRadialBlocks;
-  const int nyq = (p->n[0] % 2 == 0) ? p->nh - 1 : -1;
+  const int nyq = (p->n[p->fold] % 2 == 0 && p->s[p->fold] > 1) ? p->s[p->fold] - 1 : -1;
   const size_t lds = sizeof(double) * 4 * (size_t)nsh;
   if (p->elem == 8)
     hipLaunchKernelGGL((k_radial_bin<double>), nb, kNT, lds, s, (const cpx<double>*)f,
-                       (const cpx<double>*)g, p->M, ncomp, (unsigned)p->nh, (unsigned)p->n[1],
-                       nrows, nyq, tables_of(p), dk, nsh, part);
+                       (const cpx<double>*)g, p->M, ncomp, (unsigned)p->s[0], (unsigned)p->s[1],
+                       nrows, p->fold, nyq, tables_of(p), dk, nsh, part);
   else
     hipLaunchKernelGGL((k_radial_bin<float>), nb, kNT, lds, s, (const cpx<float>*)f,
-                       (const cpx<float>*)g, p->M, ncomp, (unsigned)p->nh, (unsigned)p->n[1],
-                       nrows, nyq, tables_of(p), dk, nsh, part);
+                       (const cpx<float>*)g, p->M, ncomp, (unsigned)p->s[0], (unsigned)p->s[1],
+                       nrows, p->fold, nyq, tables_of(p), dk, nsh, part);
   gbg_status st = check_launch("radial spectrum bins");
   if (st) return st;
   const double N = (double)p->n[0] * (double)p->n[1] * (double)p->n[2];
@@ -328,31 +341,47 @@ size_t radial_work_bytes(const gbg_plan* p) {
   return sizeof(double) * ((size_t)kRadialBlocks * n + n);
 }
 
-static int reality_planes(const gbg_plan* p) { return (p->n[0] % 2 == 0 && p->nh > 1) ? 2 : 1; }
+// the self-conjugate planes of the plan's folded axis
+static Planes reality_planes(const gbg_plan* p) {
+  const int f = p->fold, a = f == 0 ? 1 : 0, b = f == 2 ? 1 : 2;
+  size_t stride[3];
+  stride[0] = 1;
+  stride[1] = (size_t)p->s[0];
+  stride[2] = (size_t)p->s[0] * p->s[1];
+  Planes pl;
+  pl.na = (unsigned)p->s[a];
+  pl.nb = (unsigned)p->s[b];
+  pl.sa = stride[a];
+  pl.sb = stride[b];
+  pl.sf = stride[f];
+  pl.last = (unsigned)p->s[f] - 1;
+  pl.nplanes = (p->n[f] % 2 == 0 && p->s[f] > 1) ? 2 : 1;
+  return pl;
+}
 
 gbg_status op_reality_project(gbg_plan* p, void* spec, int ncomp, hipStream_t s) {
-  const int np = reality_planes(p);
-  const size_t items = (size_t)p->n[1] * p->n[2] * ncomp * np;
+  const Planes pl = reality_planes(p);
+  const size_t items = (size_t)pl.na * pl.nb * ncomp * pl.nplanes;
   if (p->elem == 8)
     hipLaunchKernelGGL((k_reality_project<double>), grid_of(items), kNT, 0, s, (cpx<double>*)spec,
-                       p->M, (unsigned)p->nh, (unsigned)p->n[1], (unsigned)p->n[2], ncomp, np);
+                       p->M, pl, ncomp);
   else
     hipLaunchKernelGGL((k_reality_project<float>), grid_of(items), kNT, 0, s, (cpx<float>*)spec,
-                       p->M, (unsigned)p->nh, (unsigned)p->n[1], (unsigned)p->n[2], ncomp, np);
+                       p->M, pl, ncomp);
   return check_launch("reality projection");
 }
 
 gbg_status op_reality_residual(gbg_plan* p, const void* spec, int ncomp, double* out_dev,
                                hipStream_t s) {
-  const int np = reality_planes(p);
-  const size_t items = (size_t)p->n[1] * p->n[2] * ncomp * np;
+  const Planes pl = reality_planes(p);
+  const size_t items = (size_t)pl.na * pl.nb * ncomp * pl.nplanes;
   const unsigned nb = grid_of(items) < (unsigned)kRedBlocks ? grid_of(items) : kRedBlocks;
   if (p->elem == 8)
     hipLaunchKernelGGL((k_reality_residual<double>), nb, kNT, 0, s, (const cpx<double>*)spec, p->M,
-                       (unsigned)p->nh, (unsigned)p->n[1], (unsigned)p->n[2], ncomp, np, p->red_part);
+                       pl, ncomp, p->red_part);
   else
     hipLaunchKernelGGL((k_reality_residual<float>), nb, kNT, 0, s, (const cpx<float>*)spec, p->M,
-                       (unsigned)p->nh, (unsigned)p->n[1], (unsigned)p->n[2], ncomp, np, p->red_part);
+                       pl, ncomp, p->red_part);
   gbg_status st = check_launch("reality residual");
   if (st) return st;
   hipLaunchKernelGGL(k_max_final, 1, kNT, 0, s, p->red_part, (int)nb, out_dev);

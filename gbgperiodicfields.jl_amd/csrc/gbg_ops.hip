@@ -380,33 +380,46 @@ __global__ __launch_bounds__(kOpNT) void k_multiply(T* dst, const T* a, const T*
 }
 
 // ---- Parseval inner, FP64 accumulation in a fixed order --------------------------------
-// sum_c sum_modes w(kx) Re(f conj g), w = 1 on kx = 0 and the even Nyquist plane,
-// else 2; result / N^2   (src/Reductions.jl:883-925, weight :956-965)
+// sum_c sum_modes w(k_f) Re(f conj g), w = 1 on the folded axis's k_f = 0 plane and its
+// even Nyquist plane, else 2; result / N^2   (src/Reductions.jl:883-925, weight :956-965)
 // Each block sums a fixed contiguous chunk; a thread takes every kOpNT-th entry
 // of it, kRedU loads in flight at a time into four accumulators (a single
 // accumulator made each step wait for the previous add and left the loads
-// unoverlapped).  The stored length M is a multiple of nh, so kx = i mod nh
-// over the flattened (component, mode) index and advances by kOpNT mod nh.
+// unoverlapped).  With st the storage stride of the folded axis and period =
+// st x its extent (a divisor of M), q = i mod period over the flattened
+// (component, mode) index advances by kOpNT mod period, and k_f = 0 is
+// q < st, the Nyquist plane nlo <= q < nhi (x folded: st = 1, q = kx).
 constexpr int kRedU = 8;
+
+struct FoldW {
+  unsigned period, st, nlo, nhi;
+};
+
+static FoldW fold_weights(const gbg_plan* p) {
+  unsigned st = 1;
+  for (int a = 0; a < p->fold; ++a) st *= (unsigned)p->s[a];
+  const unsigned ext = (unsigned)p->s[p->fold];
+  const bool even = p->n[p->fold] % 2 == 0 && ext > 1;
+  return FoldW{st * ext, st, even ? (ext - 1) * st : 0u, even ? ext * st : 0u};
+}
 
 template <typename T>
 __global__ __launch_bounds__(kOpNT) void k_inner_part(const cpx<T>* f, const cpx<T>* g, size_t M,
-                                                      int ncomp, unsigned nh, int nyq,
-                                                      double* part) {
+                                                      int ncomp, FoldW fw, double* part) {
   __shared__ double red[kOpNT];
   const size_t total = M * (size_t)ncomp;
   const size_t chunk = (total + gridDim.x - 1) / gridDim.x;
   const size_t lo = chunk * blockIdx.x;
   const size_t hi = lo + chunk < total ? lo + chunk : total;
-  const unsigned step = kOpNT % nh;
+  const unsigned step = kOpNT % fw.period;
   size_t i = lo + threadIdx.x;
-  unsigned kx = (unsigned)(i % nh);
+  unsigned q = (unsigned)(i % fw.period);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   auto add = [&](int u, const cpx<T>& a, const cpx<T>& b) {
-    const double w = (kx == 0 || (int)kx == nyq) ? 1.0 : 2.0;
+    const double w = (q < fw.st || (q >= fw.nlo && q < fw.nhi)) ? 1.0 : 2.0;
     acc[u & 3] += w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y);
-    kx += step;
-    if (kx >= nh) kx -= nh;
+    q += step;
+    if (q >= fw.period) q -= fw.period;
   };
   for (; i + (size_t)(kRedU - 1) * kOpNT < hi; i += (size_t)kRedU * kOpNT) {
     cpx<T> a[kRedU], b[kRedU];
@@ -477,7 +490,7 @@ __global__ __launch_bounds__(kOpNT) void k_inner_final(const double* part, int n
 // host launchers
 // ----------------------------------------------------------------------------------------
 static ModeIdx modes_of(gbg_plan* p) {
-  return ModeIdx{(unsigned)p->nh, (unsigned)p->n[1], 1.0 / (double)p->nh, 1.0 / (double)p->n[1]};
+  return ModeIdx{(unsigned)p->s[0], (unsigned)p->s[1], 1.0 / (double)p->s[0], 1.0 / (double)p->s[1]};
 }
 
 gbg_status op_gradient(gbg_plan* p, void* dst, const void* src, int src_order, hipStream_t s) {
@@ -637,14 +650,14 @@ gbg_status op_multiply(gbg_plan* p, void* dst, const void* a, const void* b, hip
 }
 
 gbg_status op_inner(gbg_plan* p, const void* f, const void* g, int ncomp, int slot, hipStream_t s) {
-  const int nyq = (p->n[0] % 2 == 0) ? p->nh - 1 : -1;
+  const FoldW fw = fold_weights(p);
   if (p->elem == 8)
     hipLaunchKernelGGL((k_inner_part<double>), kRedBlocks, kOpNT, 0, s, (const cpx<double>*)f,
-                       (const cpx<double>*)g, p->M, ncomp, (unsigned)p->nh, nyq,
+                       (const cpx<double>*)g, p->M, ncomp, fw,
                        p->red_part + (size_t)slot * kRedBlocks);
   else
     hipLaunchKernelGGL((k_inner_part<float>), kRedBlocks, kOpNT, 0, s, (const cpx<float>*)f,
-                       (const cpx<float>*)g, p->M, ncomp, (unsigned)p->nh, nyq,
+                       (const cpx<float>*)g, p->M, ncomp, fw,
                        p->red_part + (size_t)slot * kRedBlocks);
   gbg_status st = check_launch("inner partials");
   if (st) return st;

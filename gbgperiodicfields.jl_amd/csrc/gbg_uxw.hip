@@ -1325,8 +1325,9 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   return GBG_OK;
 }
 
+// the fused passes take x as the folded axis (the default layout)
 static bool uxw_fusable(const gbg_plan* p, unsigned flags) {
-  return !(flags & GBG_UXW_UNFUSED) && !p->generic_only && fast_len(p->n[0]) &&
+  return !(flags & GBG_UXW_UNFUSED) && !p->generic_only && p->fold == 0 && fast_len(p->n[0]) &&
          fast_len(p->n[1]) && fast_len(p->n[2]);
 }
 
@@ -1444,7 +1445,8 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
 }
 
 static bool fusable_grid(const gbg_plan* p) {
-  return !p->generic_only && fast_len(p->n[0]) && fast_len(p->n[1]) && fast_len(p->n[2]);
+  return !p->generic_only && p->fold == 0 && fast_len(p->n[0]) && fast_len(p->n[1]) &&
+         fast_len(p->n[2]);
 }
 
 size_t dealiased_product_work_bytes(gbg_plan* p, double kc) {

@@ -67,6 +67,17 @@ gbg_status gbg_stream_sync(void* stream);
  * elem_bytes = 8 (Float64) or 4 (Float32).  A plan serves any component
  * count (the reference's rank-free plans, decision 76). */
 gbg_status gbg_plan_create(const int n[3], const double L[3], int elem_bytes, gbg_plan** out);
+/* The same for a permuted RealFourierLayout(fft_dims) (src/Fields.jl:219-239):
+ * fft_dims is a permutation of {1, 2, 3} (else GBG_E_ARG, as
+ * _validate_fft_dims :122 refuses it) whose first entry is the folded axis,
+ * stored with n/2 + 1 entries; the other two keep their lengths.  Spectral
+ * buffers of such a plan are column-major over those extents.  Every
+ * transform, operator, reduction and the composed u x omega / product
+ * pipelines accept it (u x omega and the product through their composed
+ * form); the batch and the server make plans of the default layout
+ * (1, 2, 3). */
+gbg_status gbg_plan_create_layout(const int n[3], const double L[3], int elem_bytes,
+                                  const int fft_dims[3], gbg_plan** out);
 gbg_status gbg_plan_destroy(gbg_plan* plan);
 /* Sizes in elements per component (spectral complex elements, physical real
  * elements) and the plan's device bytes currently held. */

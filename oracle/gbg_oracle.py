@@ -474,7 +474,7 @@ def uxw(u_hat: np.ndarray, grid: Grid3D, kc: float | None = None, m: Multipliers
     w = to_physical(wt, grid, workers)
     n = cross(u, w)
     del u, w
-    nt = to_spectral(n, workers)
+    nt = to_spectral(n, workers, grid.fold)
     del n
     nh = helmholtz_projection(nt, m)
     nh = mask[None] * nh
@@ -493,7 +493,7 @@ def dealiased_product(a_hat: np.ndarray, b_hat: np.ndarray, grid: Grid3D, kc: fl
     mask = band_mask(m, 0.0, kc)
     a = to_physical(mask * a_hat, grid, workers)
     b = to_physical(mask * b_hat, grid, workers)
-    return mask * to_spectral(a * b, workers)
+    return mask * to_spectral(a * b, workers, grid.fold)
 
 
 # ---------------------------------------------------------------------------
