@@ -26,7 +26,7 @@ using GBGPeriodicFields
 using GBGPeriodicFields: Grid3D, AbstractFieldPhysical, AbstractFieldSpectral, ScalarFieldPhysical,
     ScalarFieldSpectral, VectorFieldPhysical, VectorFieldSpectral, TensorFieldPhysical,
     TensorFieldSpectral, FieldTransformPlans, FieldSpectralMultipliers, RadialKernel, SpectralKernel,
-    field_data, field_grid, field_tensor_order, radial_kernel
+    RealFourierLayout, field_data, field_grid, field_layout, field_tensor_order, radial_kernel
 import GBGPeriodicFields: to_spectral!, to_physical!, gradient!, symmetric_gradient!, divergence!, curl!, inverse_curl!,
     laplacian!, inverse_laplacian!, helmholtz_projection!, convolve!, spectral_band,
     mean_free_projection!, cross!, multiply!, inner, exponential_convolve!, radial_spectrum,
@@ -115,27 +115,35 @@ Adapt.adapt_storage(::Type{Array}, a::GBGHipArray) = Array(a)
 struct GBGBackend <: KernelAbstractions.GPU end
 KernelAbstractions.get_backend(::GBGHipArray) = GBGBackend()
 
-# ---- libgbg plans: one per (grid, box, precision), cached ----------------------------
+# ---- libgbg plans: one per (grid, box, precision, layout), cached ----------------------
 mutable struct GBGPlan
     handle::Ptr{Cvoid}
 end
-const PLANS = Dict{Tuple{NTuple{3,Int},NTuple{3,Float64},Int},GBGPlan}()
+const PLANS = Dict{Tuple{NTuple{3,Int},NTuple{3,Float64},Int,NTuple{3,Int}},GBGPlan}()
 const PLANS_LOCK = ReentrantLock()
 
-function plan_for(n::NTuple{3,Int}, L::NTuple{3,Float64}, ::Type{T}) where {T<:Union{Float32,Float64}}
+function plan_for(n::NTuple{3,Int}, L::NTuple{3,Float64}, ::Type{T},
+                  dims::NTuple{3,Int}=(1, 2, 3)) where {T<:Union{Float32,Float64}}
     lock(PLANS_LOCK) do
-        get!(PLANS, (n, L, sizeof(T))) do
+        get!(PLANS, (n, L, sizeof(T), dims)) do
             h = Ref{Ptr{Cvoid}}(C_NULL)
-            check(ccall((:gbg_plan_create, libgbg), Cint,
-                        (Ptr{Cint}, Ptr{Cdouble}, Cint, Ref{Ptr{Cvoid}}),
-                        Cint[n...], Cdouble[L...], sizeof(T), h))
+            if dims == (1, 2, 3)
+                check(ccall((:gbg_plan_create, libgbg), Cint,
+                            (Ptr{Cint}, Ptr{Cdouble}, Cint, Ref{Ptr{Cvoid}}),
+                            Cint[n...], Cdouble[L...], sizeof(T), h))
+            else
+                check(ccall((:gbg_plan_create_layout, libgbg), Cint,
+                            (Ptr{Cint}, Ptr{Cdouble}, Cint, Ptr{Cint}, Ref{Ptr{Cvoid}}),
+                            Cint[n...], Cdouble[L...], sizeof(T), Cint[dims...], h))
+            end
             p = GBGPlan(h[])
             finalizer(x -> ccall((:gbg_plan_destroy, libgbg), Cint, (Ptr{Cvoid},), x.handle), p)
             p
         end
     end
 end
-plan_for(grid::Grid3D{T}) where {T} = plan_for(grid.grid_size, Float64.(grid.domain_size), T)
+plan_for(grid::Grid3D{T}, layout::RealFourierLayout=RealFourierLayout()) where {T} =
+    plan_for(grid.grid_size, Float64.(grid.domain_size), T, layout.fft_dims)
 
 ncomp(f) = 3^field_tensor_order(f)
 dptr(f) = Ptr{Cvoid}(pointer(field_data(f)))
@@ -153,19 +161,22 @@ struct GBGInversePlan{T} <: AbstractFFTs.Plan{Complex{T}}
     plan::GBGPlan
     n::NTuple{3,Int}
 end
+# region = fft_dims: any permutation of (1, 2, 3), its first axis folded
 function check_dims(region)
-    Tuple(region) == (1, 2, 3) ||
-        throw(ArgumentError("libgbg folds the first axis (fft_dims = (1, 2, 3)); got $(Tuple(region))"))
+    r = Tuple(Int.(collect(region)))
+    (length(r) == 3 && Fields._validate_fft_dims(r)) ||
+        throw(ArgumentError("fft_dims must be a permutation of (1, 2, 3), got $(Tuple(region))"))
+    r
 end
 function AbstractFFTs.plan_rfft(x::GBGHipArray{T,3}, region; kws...) where {T<:Union{Float32,Float64}}
-    check_dims(region)
-    GBGForwardPlan{T}(plan_for(size(x), (2π, 2π, 2π), T), size(x))
+    dims = check_dims(region)
+    GBGForwardPlan{T}(plan_for(size(x), (2π, 2π, 2π), T, dims), size(x))
 end
 function AbstractFFTs.plan_irfft(x::GBGHipArray{Complex{T},3}, d::Integer, region;
                                  kws...) where {T<:Union{Float32,Float64}}
-    check_dims(region)
-    n = (Int(d), size(x, 2), size(x, 3))
-    GBGInversePlan{T}(plan_for(n, (2π, 2π, 2π), T), n)
+    dims = check_dims(region)
+    n = ntuple(a -> a == dims[1] ? Int(d) : size(x, a), 3)
+    GBGInversePlan{T}(plan_for(n, (2π, 2π, 2π), T, dims), n)
 end
 function LinearAlgebra.mul!(y::GBGHipArray{Complex{T},3}, p::GBGForwardPlan{T},
                             x::GBGHipArray{T,3}) where {T}
@@ -188,7 +199,7 @@ function to_spectral!(spectral::AbstractFieldSpectral{T,R}, physical::AbstractFi
     Fields._check_field_matches_context(spectral, plans)
     Fields._check_field_matches_context(physical, plans)
     check(ccall((:gbg_to_spectral, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
-                plan_for(plans.grid).handle, dptr(spectral), dptr(physical), ncomp(physical), C_NULL))
+                plan_for(plans.grid, plans.fourier_layout).handle, dptr(spectral), dptr(physical), ncomp(physical), C_NULL))
     spectral
 end
 function to_physical!(physical::AbstractFieldPhysical{T,R}, spectral::AbstractFieldSpectral{T,R},
@@ -197,7 +208,7 @@ function to_physical!(physical::AbstractFieldPhysical{T,R}, spectral::AbstractFi
     Fields._check_field_matches_context(spectral, plans)
     check(ccall((:gbg_to_physical, libgbg), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
-                plan_for(plans.grid).handle, dptr(physical), dptr(spectral), ncomp(spectral), C_NULL,
+                plan_for(plans.grid, plans.fourier_layout).handle, dptr(physical), dptr(spectral), ncomp(spectral), C_NULL,
                 C_NULL))
     physical
 end
@@ -211,7 +222,7 @@ function to_physical!(physical::AbstractFieldPhysical{T,R}, spectral::AbstractFi
     Fields._check_field_matches_context(spectral_scratch, plans)
     check(ccall((:gbg_to_physical, libgbg), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
-                plan_for(plans.grid).handle, dptr(physical), dptr(spectral), ncomp(spectral),
+                plan_for(plans.grid, plans.fourier_layout).handle, dptr(physical), dptr(spectral), ncomp(spectral),
                 dptr(spectral_scratch), C_NULL))
     physical
 end
@@ -229,7 +240,7 @@ function plan_of(m::GBGMultipliers, fields...)
     for f in fields
         Fields._check_multipliers_compatibility(f, m)
     end
-    plan_for(m.grid)
+    plan_for(m.grid, m.fourier_layout)
 end
 
 function gradient!(destination::VectorFieldSpectral{T}, field::ScalarFieldSpectral{T},
@@ -431,7 +442,7 @@ const HipSpec{T} = Union{ScalarFieldSpectral{T,<:GBGHipArray},VectorFieldSpectra
                          TensorFieldSpectral{2,T,<:GBGHipArray}}
 const HipPhys{T} = Union{ScalarFieldPhysical{T,<:GBGHipArray},VectorFieldPhysical{T,<:GBGHipArray},
                          TensorFieldPhysical{2,T,<:GBGHipArray}}
-plan_of(f) = plan_for(field_grid(f))
+plan_of(f) = plan_for(field_grid(f), field_layout(f))
 
 function mean_free_projection!(field::HipSpec{T}) where {T<:AbstractFloat}
     check(ccall((:gbg_mean_free, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}),
@@ -539,7 +550,7 @@ play on the host (src/Fields.jl:1352-1362); returns forward + inverse ms.
 function autotune!(plans::GBGTransformPlans)
     ms = Ref{Cdouble}(0.0)
     check(ccall((:gbg_plan_autotune, libgbg), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Cdouble}),
-                plan_for(plans.grid).handle, C_NULL, ms))
+                plan_for(plans.grid, plans.fourier_layout).handle, C_NULL, ms))
     return ms[]
 end
 
