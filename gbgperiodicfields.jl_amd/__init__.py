@@ -36,7 +36,7 @@ __all__ = [
     "divergence_", "curl_", "curl", "inverse_curl_", "laplacian_", "inverse_laplacian_",
     "helmholtz_projection_", "spectral_cutoff", "spectral_band", "convolve_",
     "mean_free_projection_", "cross_", "multiply_", "inner", "l2norm", "uxw_",
-    "dealiased_product_", "dealias_cutoff", "device_count",
+    "dealiased_product_", "dealias_cutoff", "device_count", "uxw_combine_",
 ]
 
 
@@ -686,6 +686,38 @@ def uxw_(n_hat: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | Non
     check(load().gbg_uxw(plans.handle, u_hat.data, n_hat.data, float(kc), None, None, flags,
                          _sh(stream)))
     return None
+
+
+def uxw_combine_(dst: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | None = None, *,
+                 n_hat: Field | None = None, a: Field | None = None, alpha: float = 1.0,
+                 b: Field | None = None, beta: float = 0.0, delta: float = 0.0, gamma: float = 0.0,
+                 coefficient: float = 0.0, fused: bool = True, reductions: bool = True,
+                 stream=None):
+    """u x omega with the integrating-factor step of ``step!``
+    (``examples/HIST_CPU.jl:267-282``) folded into its last pass:
+    ``dst = exp(c|k|^2) (alpha a + beta b + delta N^) + gamma N^``, and N^ into
+    ``n_hat`` when given.  Returns ``(E, Z)`` as ``uxw_``."""
+    _need(u_hat, "spectral", 1)
+    _need(dst, "spectral", 1)
+    _with_plan(u_hat, plans)
+    _same_domain(dst, u_hat)
+    for f in (n_hat, a, b):
+        if f is not None:
+            _need(f, "spectral", 1)
+            _same_domain(f, u_hat)
+    if dst is u_hat or (n_hat is not None and (n_hat is u_hat or n_hat is dst)):
+        raise AliasError(4, "dst and n_hat must not be u_hat or each other")
+    if kc is None:
+        kc = dealias_cutoff(plans.grid.grid_size[0], plans.grid.domain_size[0])
+    cmb = _lib.GbgUxwCombination(dst.data, None if a is None else a.data, float(alpha),
+                                 None if b is None else b.data, float(beta), float(delta),
+                                 float(gamma), float(coefficient))
+    flags = 0 if fused else _lib.GBG_UXW_UNFUSED
+    E, Z = C.c_double(0.0), C.c_double(0.0)
+    check(load().gbg_uxw_combine(plans.handle, u_hat.data, None if n_hat is None else n_hat.data,
+                                 float(kc), C.byref(cmb), C.byref(E) if reductions else None,
+                                 C.byref(Z) if reductions else None, flags, _sh(stream)))
+    return (E.value, Z.value) if reductions else None
 
 
 class Batch:

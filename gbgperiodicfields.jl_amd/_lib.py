@@ -26,6 +26,14 @@ class GbgJob(C.Structure):
                 ("in1", C.c_void_p), ("out", C.c_void_p), ("kc", C.c_double)]
 
 
+class GbgUxwCombination(C.Structure):
+    """struct gbg_uxw_combination of include/gbg.h."""
+
+    _fields_ = [("dst", C.c_void_p), ("a", C.c_void_p), ("alpha", C.c_double), ("b", C.c_void_p),
+                ("beta", C.c_double), ("delta", C.c_double), ("gamma", C.c_double),
+                ("coefficient", C.c_double)]
+
+
 vp = C.c_void_p
 st = C.c_int
 sz = C.c_size_t
@@ -76,6 +84,7 @@ SIGNATURES = {
     "gbg_reality_residual": (st, [vp, vp, i32, dp, vp]),
     "gbg_reality_projection": (st, [vp, vp, vp, i32, vp]),
     "gbg_uxw": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
+    "gbg_uxw_combine": (st, [vp, vp, vp, dbl, C.POINTER(GbgUxwCombination), dp, dp, C.c_uint, vp]),
     "gbg_uxw_noalloc": (st, [vp, vp, vp, dbl, dp, dp, C.c_uint, vp]),
     "gbg_dealiased_product": (st, [vp, vp, vp, vp, dbl, vp]),
     "gbg_plan_autotune": (st, [vp, vp, dp]),

@@ -833,6 +833,46 @@ gbg_status gbg_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, doubl
   GBG_GUARD_END
 }
 
+gbg_status gbg_uxw_combine(gbg_plan* p, const void* u_hat, void* n_hat, double kc,
+                           const gbg_uxw_combination* comb, double* energy, double* enstrophy,
+                           unsigned flags, void* stream) {
+  GBG_GUARD_BEGIN
+  gbg_status st = need_plan(p);
+  if (st) return st;
+  if (!u_hat || !comb || !comb->dst) return fail(GBG_E_ARG, "null field buffer");
+  if (!std::isfinite(kc) || kc < 0) return fail(GBG_E_ARG, "cutoff must be finite and >= 0");
+  const double w[5] = {comb->alpha, comb->beta, comb->delta, comb->gamma, comb->coefficient};
+  for (double v : w)
+    if (!std::isfinite(v)) return fail(GBG_E_ARG, "combination weights must be finite");
+  const size_t b = 3 * p->M * 2 * p->elem;
+  const void* dst = comb->dst;
+  if (overlaps(dst, b, u_hat, b)) return fail(GBG_E_ALIAS, "dst must not overlap u_hat");
+  if (n_hat) {
+    if (overlaps(n_hat, b, u_hat, b)) return fail(GBG_E_ALIAS, "u_hat and n_hat must not overlap");
+    if (overlaps(n_hat, b, dst, b)) return fail(GBG_E_ALIAS, "n_hat must not overlap dst");
+    if ((comb->a && overlaps(n_hat, b, comb->a, b)) || (comb->b && overlaps(n_hat, b, comb->b, b)))
+      return fail(GBG_E_ALIAS, "n_hat must not overlap a or b");
+  }
+  // dst may be a or b itself (each mode is read before it is written), not a
+  // shifted view of either
+  for (const void* x : {comb->a, comb->b})
+    if (x && x != dst && overlaps(dst, b, x, b))
+      return fail(GBG_E_ALIAS, "dst overlaps a or b without being it");
+  UxwComb cb;
+  cb.dst = comb->dst;
+  cb.a = comb->a;
+  cb.b = comb->b;
+  cb.alpha = comb->alpha;
+  cb.beta = comb->beta;
+  cb.delta = comb->delta;
+  cb.gamma = comb->gamma;
+  cb.coef = comb->coefficient;
+  hipStream_t s = as_stream(stream);
+  if ((st = pipe_uxw_comb(p, u_hat, n_hat, kc, cb, flags, s))) return st;
+  return uxw_reductions(p, energy, enstrophy, s);
+  GBG_GUARD_END
+}
+
 gbg_status gbg_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, const void* b_hat,
                                  double kc, void* stream) {
   GBG_GUARD_BEGIN

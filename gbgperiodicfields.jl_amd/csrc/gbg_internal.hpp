@@ -237,6 +237,20 @@ gbg_status pipe_dealiased_product(gbg_plan* p, void* p_hat, const void* a_hat, c
 // the device (no work area, tile set or stream)
 gbg_status pipe_uxw_in(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
                        void* work, size_t work_bytes, hipStream_t s);
+// u x omega followed by the integrating-factor combination
+//   dst = exp(coef |k|^2) (alpha a + beta b + delta N^) + gamma N^
+// applied as N^ is formed (in the last fused pass, or one streaming pass after
+// the composed form).  n_hat may be null (N^ then is not written); a, b may be
+// null.  The caller has checked the buffers (dst off u_hat, n_hat and the
+// inputs apart; dst may be a or b itself).
+struct UxwComb {
+  void* dst;
+  const void* a;
+  const void* b;
+  double alpha, beta, delta, gamma, coef;
+};
+gbg_status pipe_uxw_comb(gbg_plan* p, const void* u_hat, void* n_hat, double kc, const UxwComb& cb,
+                         unsigned flags, hipStream_t s);
 // work-area bytes pipe_uxw needs on this plan's grid for cutoff kc (reads
 // only the plan's n, L, nh, M, P, elem and generic_only)
 size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags);

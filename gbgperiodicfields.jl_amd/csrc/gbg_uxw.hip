@@ -33,39 +33,10 @@
 
 namespace gbg {
 
-// ============================================================================
-// composed path
-// ============================================================================
-// w: 2 (3 M s_c) + 2 (3 P s) bytes of work area
+// The composed path (the standalone transforms and operators in sequence) is
+// defined after the fused passes, whose combination kernel it shares.
 static gbg_status uxw_composed(gbg_plan* p, const void* u_hat, void* n_hat, double kc, char* w,
-                               hipStream_t s) {
-  const size_t cs = (size_t)p->elem * 2;  // complex element bytes
-  const size_t rs = (size_t)p->elem;      // real element bytes
-  const size_t spec3 = 3 * p->M * cs, phys3 = 3 * p->P * rs;
-  gbg_status st = GBG_OK;
-  void* ut = w;
-  void* wt = w + spec3;
-  void* u = w + 2 * spec3;
-  void* om = w + 2 * spec3 + phys3;
-  prof_begin(p, s);
-  if ((st = op_convolve_band(p, ut, u_hat, 3, 0.0, kc, 1.0, 0.0, s))) return st;
-  if ((st = op_curl(p, wt, ut, 0, s))) return st;
-  if ((st = op_inner(p, ut, ut, 3, 0, s))) return st;
-  if ((st = op_inner(p, wt, wt, 3, 1, s))) return st;
-  prof_mark(p, s);
-  if ((st = inv_transform(p, u, ut, ut, 3, s))) return st;
-  if ((st = inv_transform(p, om, wt, wt, 3, s))) return st;
-  prof_mark(p, s);
-  if ((st = op_cross(p, u, u, om, s))) return st;
-  prof_mark(p, s);
-  if ((st = fwd_transform(p, n_hat, u, 3, s))) return st;
-  prof_mark(p, s);
-  if ((st = op_helmholtz(p, n_hat, n_hat, 0, 0.0, 0.0, 1.0, 0.0, s))) return st;
-  if ((st = op_convolve_band(p, n_hat, n_hat, 3, 0.0, kc, 1.0, 0.0, s))) return st;
-  if ((st = op_mean_free(p, n_hat, 3, s))) return st;
-  prof_mark(p, s);
-  return GBG_OK;
-}
+                               hipStream_t s, const UxwComb* cb = nullptr);
 
 // ============================================================================
 // fused path
@@ -808,10 +779,39 @@ constexpr bool z_forward_persist() {
   return sizeof(T) == 8 && (N == 128 || N == 512);
 }
 
-template <typename T, int N, int NC>
+// The integrating-factor combination K5 can apply to N^ before it leaves the
+// registers (the step either side of N^ in an integrating-factor RK2 update,
+// examples/HIST_CPU.jl:267-282):
+//   dst = e (alpha a + beta b + delta N^) + gamma N^,  e = exp(c |k|^2),
+// |k|^2 = (k1^2 + k2^2) + k3^2 from the squared family in FP64 and e rounded to
+// T, as the standalone exponential kernel forms it (_exponential_exponent,
+// src/Convolution.jl:1289-1294).  a, b may be null (their terms dropped); with
+// store_n = 0 the output spectrum N^ itself is not written.
+template <typename T>
+struct Epi {
+  cpx<T>* dst;
+  const cpx<T>* a;
+  const cpx<T>* b;
+  T alpha, beta, delta, gamma;
+  double coef;
+  int store_n;
+};
+
+// the combination at entry o of component c (o < M) for the mode of squared
+// wavenumber q
+template <typename T>
+__device__ __forceinline__ void epi_store(const Epi<T>& ep, long long co, double q, cpx<T> n) {
+  const T e = T(exp(ep.coef * q));
+  cpx<T> acc = cscale(n, ep.delta);
+  if (ep.a) acc = cadd(cscale(ep.a[co], ep.alpha), acc);
+  if (ep.b) acc = cadd(acc, cscale(ep.b[co], ep.beta));
+  ep.dst[co] = cadd(cscale(acc, e), cscale(n, ep.gamma));
+}
+
+template <typename T, int N, int NC, bool EPI = false>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ w1, cpx<T>* __restrict__ nh, Geo g, Tables tb,
-    const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles) {
+    const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles, Epi<T> ep) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
@@ -857,7 +857,15 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int m = 0; m < E; ++m) nh[c * g.M + (long long)(t + TP * m) * plane + p5] = mk<T>(T(0), T(0));
+          for (int m = 0; m < E; ++m) {
+            const long long co = c * g.M + (long long)(t + TP * m) * plane + p5;
+            if constexpr (EPI) {
+              if (ep.store_n) nh[co] = mk<T>(T(0), T(0));
+              epi_store(ep, co, qxy + tb.ks3[t + TP * m], mk<T>(T(0), T(0)));
+            } else {
+              nh[co] = mk<T>(T(0), T(0));
+            }
+          }
       }
       return;
     }
@@ -922,9 +930,24 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
         const long long o = (long long)kz * plane + p5;
         // component-wise selects (a select of whole complex values is lowered
         // through a private array in FP64)
-        nh[o] = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
-        nh[g.M + o] = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
-        nh[2 * g.M + o] = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+        const cpx<T> r1 = mk<T>(keep[m] ? p1.x : T(0), keep[m] ? p1.y : T(0));
+        const cpx<T> r2 = mk<T>(keep[m] ? p2.x : T(0), keep[m] ? p2.y : T(0));
+        const cpx<T> r3 = mk<T>(keep[m] ? p3.x : T(0), keep[m] ? p3.y : T(0));
+        if constexpr (EPI) {
+          if (ep.store_n) {
+            nh[o] = r1;
+            nh[g.M + o] = r2;
+            nh[2 * g.M + o] = r3;
+          }
+          const double q = qxy + tb.ks3[kz];
+          epi_store(ep, o, q, r1);
+          epi_store(ep, g.M + o, q, r2);
+          epi_store(ep, 2 * g.M + o, q, r3);
+        } else {
+          nh[o] = r1;
+          nh[g.M + o] = r2;
+          nh[2 * g.M + o] = r3;
+        }
       }
     }
   };
@@ -968,6 +991,54 @@ __global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
     for (int i = 0; i < R; ++i)
       if (z[i]) out[base + i * 256] = zero;
   }
+}
+
+// The same output tiles under the integrating-factor combination: N^ is zero
+// there, so dst = e (alpha a + beta b) (and N^ = 0 when it is stored).  Plane
+// pl is (component pl / nz, kz = pl mod nz); entry p5 of a plane is (kx, ky) =
+// (p5 mod nh, p5 / nh).
+template <typename T>
+__global__ __launch_bounds__(256) void k_uxw_zero_tiles_epi(cpx<T>* __restrict__ nh, Epi<T> ep,
+                                                            const int* __restrict__ flag, int b,
+                                                            long long plane, int nplanes, int nz,
+                                                            int nhx, Tables tb) {
+  constexpr int R = kZeroRun / 256;
+  const long long base = (long long)blockIdx.x * kZeroRun + threadIdx.x;
+  bool z[R];
+  double qxy[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const long long p5 = base + i * 256;
+    z[i] = p5 < plane && !flag[p5 / b];
+    const int ky = z[i] ? (int)(p5 / nhx) : 0, kx = z[i] ? (int)(p5 - (long long)ky * nhx) : 0;
+    qxy[i] = tb.ks1[kx] + tb.ks2[ky];
+  }
+  const int p0 = blockIdx.y * kZeroPlanes;
+  const int p1 = p0 + kZeroPlanes < nplanes ? p0 + kZeroPlanes : nplanes;
+  for (int pl = p0; pl < p1; ++pl) {
+    const double q3 = tb.ks3[pl % nz];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (!z[i]) continue;
+      const long long co = (long long)pl * plane + base + i * 256;
+      if (ep.store_n) nh[co] = mk<T>(T(0), T(0));
+      epi_store(ep, co, qxy[i] + q3, mk<T>(T(0), T(0)));
+    }
+  }
+}
+
+// The combination over every mode from a stored N^ (the composed path):
+// one thread per mode, the three components.
+template <typename T>
+__global__ __launch_bounds__(256) void k_uxw_epilogue(const cpx<T>* __restrict__ n, Epi<T> ep,
+                                                      long long M, int s0, int s1, Tables tb) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= M) return;
+  const long long r = idx / s0;
+  const int kx = (int)(idx - r * s0), ky = (int)(r % s1), kz = (int)(r / s1);
+  const double q = (tb.ks1[kx] + tb.ks2[ky]) + tb.ks3[kz];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) epi_store(ep, c * M + idx, q, n[c * M + idx]);
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -1044,10 +1115,18 @@ static void launch_x(int var, const Geo& g, const cpx<T>* W2, cpx<T>* W1, const 
 template <typename T, int N, int NC>
 static void launch_z_forward(const gbg_plan* p, const Geo& g, long long nt, const cpx<T>* w,
                              cpx<T>* out, const Tables& tb, const cpx<T>* twz, const int* list,
-                             hipStream_t s) {
+                             hipStream_t s, const Epi<T>* ep = nullptr) {
   using C = Cfg<T, N>;
-  hipLaunchKernelGGL((k_uxw_z_forward<T, N, NC>), dim3(pass_grid(p, nt, z_forward_persist<T, N>())),
-                     dim3(C::NT), 0, s, w, out, g, tb, twz, list, nt);
+  const dim3 grid(pass_grid(p, nt, z_forward_persist<T, N>()));
+  if constexpr (NC == 3) {
+    if (ep) {
+      hipLaunchKernelGGL((k_uxw_z_forward<T, N, NC, true>), grid, dim3(C::NT), 0, s, w, out, g, tb,
+                         twz, list, nt, *ep);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_uxw_z_forward<T, N, NC>), grid, dim3(C::NT), 0, s, w, out, g, tb, twz, list,
+                     nt, Epi<T>{});
 }
 
 static bool fast_len(int n) {
@@ -1204,13 +1283,18 @@ static bool overlaps(const void* a, size_t abytes, const void* b, size_t bbytes)
 
 template <typename T>
 static void launch_zero_tiles(const Geo& g, const TileSet* ts, int b, int nc, cpx<T>* out,
-                              hipStream_t s) {
+                              hipStream_t s, const Epi<T>* ep = nullptr, const Tables* tb = nullptr) {
   // planes (component c, kz) are consecutive: c M + kz plane = (c nz + kz) plane
   const long long plane = (long long)g.ny * g.nh;
   const int np = nc * g.nz;
   const dim3 zg((unsigned)((plane + kZeroRun - 1) / kZeroRun),
                 (unsigned)((np + kZeroPlanes - 1) / kZeroPlanes));
-  hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, b, plane, np);
+  if (ep)
+    hipLaunchKernelGGL((k_uxw_zero_tiles_epi<T>), zg, dim3(256), 0, s, out, *ep, ts->dev + ts->nd, b,
+                       plane, np, g.nz, g.nh, *tb);
+  else
+    hipLaunchKernelGGL((k_uxw_zero_tiles<T>), zg, dim3(256), 0, s, out, ts->dev + ts->nd, b, plane,
+                       np);
 }
 
 // fork point: recorded on s before the first pass
@@ -1238,13 +1322,14 @@ static const TileSet* zero_tiles_fork(gbg_plan* p, const Geo& g, double kc, int 
 // launch so that pass's workgroups are dispatched first and the zero writes
 // fill the room they leave
 template <typename T>
-static bool zero_tiles_launch(gbg_plan* p, const Geo& g, const TileSet* ts, int nc, cpx<T>* out) {
+static bool zero_tiles_launch(gbg_plan* p, const Geo& g, const TileSet* ts, int nc, cpx<T>* out,
+                              const Epi<T>* ep = nullptr, const Tables* tb = nullptr) {
   if (!ts) return false;
   if (hipStreamWaitEvent(p->aux, p->aux_fork, 0) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
-  launch_zero_tiles<T>(g, ts, ts->b, nc, out, p->aux);
+  launch_zero_tiles<T>(g, ts, ts->b, nc, out, p->aux, ep, tb);
   if (hipEventRecord(p->aux_join, p->aux) == hipSuccess) return true;
   // no join point: let the second stream drain, and the last pass writes the
   // zero tiles on s as well
@@ -1263,7 +1348,8 @@ static gbg_status zero_tiles_join(gbg_plan* p, hipStream_t s) {
 // when the plan already caches one for this cutoff.
 template <typename T>
 static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
-                            double kc, char* base, bool noalloc, hipStream_t s) {
+                            double kc, char* base, bool noalloc, hipStream_t s,
+                            const Epi<T>* ep = nullptr) {
   const size_t cb = sizeof(cpx<T>);
   // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
   // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields
@@ -1278,9 +1364,12 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   const cpx<T>* twy = (const cpx<T>*)p->tw[1];
   const cpx<T>* twz = (const cpx<T>*)p->tw[2];
   const double P = (double)g.nx * g.ny * g.nz;
+  // with the combination the zero tiles write dst (the caller keeps dst off
+  // u^); N^ itself may be absent
+  cpx<T>* zout = ep ? ep->dst : out;
   const TileSet* fork =
       noalloc ? nullptr
-              : zero_tiles_fork<T>(p, g, kc, 3, out, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
+              : zero_tiles_fork<T>(p, g, kc, 3, zout, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
   prof_begin(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
@@ -1288,7 +1377,7 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
   });
   if ((st = check_launch("uxw K1"))) return st;
-  const bool forked = zero_tiles_launch<T>(p, g, fork, 3, out);
+  const bool forked = zero_tiles_launch<T>(p, g, fork, 3, out, ep, &tb);
   prof_mark(p, s);
   with_len(g.ny, [&](auto L) {
     launch_y<T, decltype(L)::value, 1, 6>(g, W1, W2, tb, twy, s);
@@ -1312,12 +1401,12 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     const TileSet* ts = tile_set(p, g, C::B, kc, s, !noalloc);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
-      launch_z_forward<T, N, 3>(p, g, nt, W2, out, tb, twz, nullptr, s);
+      launch_z_forward<T, N, 3>(p, g, nt, W2, out, tb, twz, nullptr, s, ep);
       return;
     }
-    if (!forked && ts->nd < ts->nt) launch_zero_tiles<T>(g, ts, C::B, 3, out, s);
+    if (!forked && ts->nd < ts->nt) launch_zero_tiles<T>(g, ts, C::B, 3, out, s, ep, &tb);
     if (ts->nd > 0)
-      launch_z_forward<T, N, 3>(p, g, ts->nd, W2, out, tb, twz, ts->dev, s);
+      launch_z_forward<T, N, 3>(p, g, ts->nd, W2, out, tb, twz, ts->dev, s, ep);
   });
   if ((st = check_launch("uxw K5"))) return st;
   if (forked && (st = zero_tiles_join(p, s))) return st;
@@ -1342,15 +1431,89 @@ size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags) {
   return b1 + b2 + sizeof(double) * 2 * ((size_t)g.nyc * g.nhp + fast::kXcd);
 }
 
+template <typename T>
+static Epi<T> make_epi(const UxwComb& c, bool store_n) {
+  Epi<T> e;
+  e.dst = (cpx<T>*)c.dst;
+  e.a = (const cpx<T>*)c.a;
+  e.b = (const cpx<T>*)c.b;
+  e.alpha = T(c.alpha);
+  e.beta = T(c.beta);
+  e.delta = T(c.delta);
+  e.gamma = T(c.gamma);
+  e.coef = c.coef;
+  e.store_n = store_n ? 1 : 0;
+  return e;
+}
+
+// ============================================================================
+// composed path
+// ============================================================================
+// w: 2 (3 M s_c) + 2 (3 P s) bytes of work area.  With a combination (cb) and
+// no n_hat, N^ is formed in the first 3 M s_c of w (free by then) and the
+// combination is applied to it in one more streaming pass.
+static gbg_status uxw_composed(gbg_plan* p, const void* u_hat, void* n_hat, double kc, char* w,
+                               hipStream_t s, const UxwComb* cb) {
+  const size_t cs = (size_t)p->elem * 2;  // complex element bytes
+  const size_t rs = (size_t)p->elem;      // real element bytes
+  const size_t spec3 = 3 * p->M * cs, phys3 = 3 * p->P * rs;
+  gbg_status st = GBG_OK;
+  void* ut = w;
+  void* wt = w + spec3;
+  void* u = w + 2 * spec3;
+  void* om = w + 2 * spec3 + phys3;
+  prof_begin(p, s);
+  if ((st = op_convolve_band(p, ut, u_hat, 3, 0.0, kc, 1.0, 0.0, s))) return st;
+  if ((st = op_curl(p, wt, ut, 0, s))) return st;
+  if ((st = op_inner(p, ut, ut, 3, 0, s))) return st;
+  if ((st = op_inner(p, wt, wt, 3, 1, s))) return st;
+  prof_mark(p, s);
+  if ((st = inv_transform(p, u, ut, ut, 3, s))) return st;
+  if ((st = inv_transform(p, om, wt, wt, 3, s))) return st;
+  prof_mark(p, s);
+  if ((st = op_cross(p, u, u, om, s))) return st;
+  prof_mark(p, s);
+  void* nt = n_hat ? n_hat : ut;
+  if ((st = fwd_transform(p, nt, u, 3, s))) return st;
+  prof_mark(p, s);
+  if ((st = op_helmholtz(p, nt, nt, 0, 0.0, 0.0, 1.0, 0.0, s))) return st;
+  if ((st = op_convolve_band(p, nt, nt, 3, 0.0, kc, 1.0, 0.0, s))) return st;
+  if ((st = op_mean_free(p, nt, 3, s))) return st;
+  if (cb) {
+    const unsigned grid = (unsigned)((p->M + 255) / 256);
+    if (p->elem == 8)
+      hipLaunchKernelGGL(k_uxw_epilogue<double>, dim3(grid), dim3(256), 0, s,
+                         (const cpx<double>*)nt, make_epi<double>(*cb, false), (long long)p->M,
+                         p->s[0], p->s[1], tables_of(p));
+    else
+      hipLaunchKernelGGL(k_uxw_epilogue<float>, dim3(grid), dim3(256), 0, s,
+                         (const cpx<float>*)nt, make_epi<float>(*cb, false), (long long)p->M,
+                         p->s[0], p->s[1], tables_of(p));
+    if ((st = check_launch("uxw combination"))) return st;
+  }
+  prof_mark(p, s);
+  return GBG_OK;
+}
+
 static gbg_status uxw_run(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,
-                          char* work, bool noalloc, hipStream_t s) {
-  if (!uxw_fusable(p, flags)) return uxw_composed(p, u_hat, n_hat, kc, work, s);
+                          char* work, bool noalloc, hipStream_t s, const UxwComb* cb = nullptr) {
+  if (!uxw_fusable(p, flags)) return uxw_composed(p, u_hat, n_hat, kc, work, s, cb);
   const Geo g = make_geo(p, kc);
-  if (p->elem == 8)
+  if (p->elem == 8) {
+    const Epi<double> e = cb ? make_epi<double>(*cb, n_hat != nullptr) : Epi<double>{};
     return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, kc, work,
-                             noalloc, s);
+                             noalloc, s, cb ? &e : nullptr);
+  }
+  const Epi<float> e = cb ? make_epi<float>(*cb, n_hat != nullptr) : Epi<float>{};
   return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, work, noalloc,
-                          s);
+                          s, cb ? &e : nullptr);
+}
+
+gbg_status pipe_uxw_comb(gbg_plan* p, const void* u_hat, void* n_hat, double kc, const UxwComb& cb,
+                         unsigned flags, hipStream_t s) {
+  gbg_status st = ensure_work(p, uxw_work_bytes(p, kc, flags));
+  if (st) return st;
+  return uxw_run(p, u_hat, n_hat, kc, flags, (char*)p->work, false, s, &cb);
 }
 
 gbg_status pipe_uxw(gbg_plan* p, const void* u_hat, void* n_hat, double kc, unsigned flags,

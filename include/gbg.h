@@ -216,6 +216,33 @@ gbg_status gbg_uxw(gbg_plan* plan, const void* u_hat, void* n_hat, double kc, do
  * array: calls that overlap in time on one device race. */
 gbg_status gbg_uxw_noalloc(gbg_plan* plan, const void* u_hat, void* n_hat, double kc,
                            double* energy, double* enstrophy, unsigned flags, void* stream);
+/* u x omega with the integrating-factor step folded into its last pass: the
+ * update on either side of N^ in an integrating-factor RK2 step
+ * (examples/HIST_CPU.jl:267-282: u* = E(u_n + dt N_1), then
+ * u_{n+1} = E u_n + dt/2 (E N_1 + N_2)) as
+ *   dst = exp(c |k|^2) (alpha a + beta b + delta N^) + gamma N^
+ * with exp(c |k|^2) the exponential kernel of exponential_convolve!
+ * (src/Convolution.jl:1289-1294, 1326-1346), |k|^2 = (k1^2 + k2^2) + k3^2.
+ * Stage 1: n_hat = N_1, a = u_n, alpha = 1, delta = dt, dst = u*.
+ * Stage 2: n_hat = NULL, a = u_n, b = N_1, alpha = 1, beta = gamma = dt/2,
+ * dst = u_n (in place).  a, b may be NULL; n_hat NULL skips writing N^.
+ * dst must not overlap u_hat or n_hat; it may be a or b itself but no other
+ * overlap of them (GBG_E_ALIAS).  energy / enstrophy as gbg_uxw.  No
+ * counterpart entry point in the reference (it composes rhs! and
+ * exponential_convolve! calls). */
+typedef struct gbg_uxw_combination {
+  void* dst;
+  const void* a;
+  double alpha;
+  const void* b;
+  double beta;
+  double delta;       /* weight of N^ inside the exponential factor */
+  double gamma;       /* weight of N^ outside it */
+  double coefficient; /* c */
+} gbg_uxw_combination;
+gbg_status gbg_uxw_combine(gbg_plan* plan, const void* u_hat, void* n_hat, double kc,
+                           const gbg_uxw_combination* comb, double* energy, double* enstrophy,
+                           unsigned flags, void* stream);
 /* Dealiased scalar product p^ = M F(F^-1(M a^) * F^-1(M b^)).  Inputs read only. */
 gbg_status gbg_dealiased_product(gbg_plan* plan, void* p_hat, const void* a_hat, const void* b_hat,
                                  double kc, void* stream);
