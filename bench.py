@@ -328,6 +328,52 @@ def run_ops(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
     return res
 
 
+def run_step(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64, nu=0.02, dt=0.01):
+    """One integrating-factor RK2 step of the consumer's step! (without forcing,
+    examples/HIST_CPU.jl:267-282) on a 512^3 solenoidal field, two ways:
+    folded -- two gbg_uxw_combine calls (stage 1 writes N_1 and u* = E(u + dt N_1),
+    stage 2 writes u <- E(u + dt/2 N_1) + dt/2 N_2 in place); standalone -- two
+    u x omega terms and three exponential-kernel passes (u* = E(u + dt N_1),
+    u <- E(u + dt/2 N_1), u <- u + dt/2 N_2), the composition of reference calls."""
+    from gbg_amd import workload
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = workload.synthetic_field(n, 2001, vector=True)
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    del u
+    k1, us, k2 = (G.Field(grid, "spectral", 1, dtype) for _ in range(3))
+    kc = G.dealias_cutoff(n[0])
+    c = -nu * dt
+
+    def folded():
+        G.uxw_combine_(us, du, plans, kc, n_hat=k1, a=du, delta=dt, coefficient=c,
+                       reductions=False, stream=stream)
+        G.uxw_combine_(du, us, plans, kc, a=du, b=k1, beta=dt / 2, gamma=dt / 2, coefficient=c,
+                       reductions=False, stream=stream)
+
+    def standalone():
+        G.uxw_(k1, du, plans, kc, reductions=False, stream=stream)
+        G.exponential_convolve_(us, 1.0, du, dt, k1, plans, coefficient=c, stream=stream)
+        G.uxw_(k2, us, plans, kc, reductions=False, stream=stream)
+        G.exponential_convolve_(du, 1.0, du, dt / 2, k1, plans, coefficient=c, stream=stream)
+        G.exponential_convolve_(du, 1.0, du, dt / 2, k2, plans, coefficient=0.0, stream=stream)
+
+    res = {}
+    for name, call in (("folded", folded), ("standalone", standalone)):
+        for _ in range(warmup):
+            call()
+        stream.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream.handle)
+        for _ in range(steps):
+            call()
+        ev.record(e1, stream.handle)
+        res[name] = {"ms_per_step": round(ev.elapsed(e0, e1) / steps, 4)}
+        print(f"[step] {name}: {res[name]}", file=sys.stderr, flush=True)
+    res["speedup"] = round(res["standalone"]["ms_per_step"] / res["folded"]["ms_per_step"], 4)
+    return res
+
+
 def run_rows(G, stream, ev, steps, warmup, n=(512, 512, 512), dtype=np.float64):
     """Each operator of the spectral path on its own at --size, timed by HIP
     events on the launch stream, with its algorithmic bytes (the fused-pass
@@ -555,12 +601,13 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3,
                     help="timed CPU-baseline terms per precision after one warm-up")
     ap.add_argument("--form", default="uxw",
-                    choices=["uxw", "batch", "serve", "hbm", "ops", "rows"],
+                    choices=["uxw", "batch", "serve", "hbm", "ops", "rows", "step"],
                     help="uxw: the headline 512^3 term (default); batch: the 96-field batched "
                          "graph form; serve: the Poisson request stream; hbm: the streaming "
                          "copy/fill rates of the device; ops: exponential kernel, radial "
                          "spectrum and reality projection at --size; rows: every transform, "
-                         "operator, product and reduction on its own at --size")
+                         "operator, product and reduction on its own at --size; step: one "
+                         "integrating-factor RK2 step at --size, folded and standalone")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device twice (a plain copy kernel of known "
                          "size, the scale for the HBM byte counters)")
@@ -608,6 +655,18 @@ def main():
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"],
                             "traffic": None},
                "ops": res}
+        print(json.dumps(out), flush=True)
+        return
+    if args.form == "step":
+        res = run_step(G, stream, ev, args.steps, args.warmup, n, dtype)
+        out = {"metric": f"{n[0]}^3 integrating-factor RK2 step (two u x omega terms and the "
+                         "exponential combinations), steps/s", "value": round(1e3 / res["folded"]["ms_per_step"], 3),
+               "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": res["folded"]["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+               "data": "synthetic (seeded k^-5/3 solenoidal spectrum, seed 2001)",
+               "config": {"workload": f"{n[0]}x{n[1]}x{n[2]} RK2 step, nu = 0.02, dt = 0.01"},
+               "step": res}
         print(json.dumps(out), flush=True)
         return
     if args.form == "rows":
