@@ -253,13 +253,13 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, const
     // chirp-z axis: one pencil of its convolution length)
     const int cap = 65536 / (2 * elem);
     if (pencil_len(p->ax[a]) + 1 > cap) {
+      const std::string msg =
+          "axis length " + std::to_string(n[a]) +
+          (p->ax[a].m ? " (a prime factor above 13: chirp-z length " + std::to_string(p->ax[a].m) + ")"
+                      : std::string()) +
+          " exceeds one LDS pencil";
       destroy_plan(p);
-      return fail(GBG_E_UNSUPPORTED,
-                  "axis length " + std::to_string(n[a]) +
-                      (p->ax[a].m ? " (a prime factor above 13: chirp-z length " +
-                                        std::to_string(p->ax[a].m) + ")"
-                                  : std::string()) +
-                      " exceeds one LDS pencil");
+      return fail(GBG_E_UNSUPPORTED, msg);
     }
   }
   p->elem = elem;
