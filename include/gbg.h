@@ -180,8 +180,9 @@ gbg_status gbg_exponential_combine(gbg_plan* plan, void* dst, double alpha, cons
 
 /* ---- reality condition (src/Fields.jl:2858-3094) ---------------------------- */
 /* reality_residual (:2918-2928): the largest |f(k) - conj f(-k)| over the
- * kx = 0 plane and, for even Nx, the Nyquist plane, every component.  Host
- * FP64 result; synchronises. */
+ * folded axis's k = 0 plane and, for an even folded length, its Nyquist
+ * plane (x for the default layout), every component.  Host FP64 result;
+ * synchronises. */
 gbg_status gbg_reality_residual(gbg_plan* plan, const void* spec, int ncomp, double* residual,
                                 void* stream);
 /* reality_projection!(dst, src) / reality_projection!(f) (:2940-3094): each
