@@ -201,7 +201,7 @@ BATCH_SHAPES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 12
                 (128, 128, 256), (256, 64, 64), (256, 256, 256)]
 
 
-def run_batch(G, stream, ev, steps, warmup, nitems=96, seed=4001):
+def run_batch(G, stream, ev, steps, warmup, nitems=96, seed=4001, nstreams=4):
     """Batched form: nitems seeded fields of uneven grids, each gradient plus
     dealiased product (scalar) or gradient plus u x w (vector), one graph
     launch per batch, FP64 and FP32."""
@@ -214,11 +214,11 @@ def run_batch(G, stream, ev, steps, warmup, nitems=96, seed=4001):
     for i, (n, k) in enumerate(zip(shapes, kinds)):
         specs.append(workload.synthetic_field(n, seed + 1 + i, vector=bool(k)))
     gen_s = time.time() - t0
-    res = {"items": nitems, "nstreams": 4, "generation_s": round(gen_s, 1),
+    res = {"items": nitems, "nstreams": nstreams, "generation_s": round(gen_s, 1),
            "mix": {f"{n[0]}x{n[1]}x{n[2]}": int(sum(1 for s in shapes if s == n)) for n in BATCH_SHAPES},
            "vector_items": int(kinds.sum())}
     for dtype, tag in ((np.float64, "f64"), (np.float32, "f32")):
-        batch = G.Batch(nstreams=4, autotune=True)
+        batch = G.Batch(nstreams=nstreams, autotune=True)
         alg = 0
         for (n, k, f) in zip(shapes, kinds, specs):
             grid = G.Grid3D(n)
@@ -608,6 +608,8 @@ def main():
                          "spectrum and reality projection at --size; rows: every transform, "
                          "operator, product and reduction on its own at --size; step: one "
                          "integrating-factor RK2 step at --size, folded and standalone")
+    ap.add_argument("--batch-streams", type=int, default=4,
+                    help="streams the batched form spreads its items over (--form batch)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device-to-device twice (a plain copy kernel of known "
                          "size, the scale for the HBM byte counters)")
@@ -629,7 +631,7 @@ def main():
     ev = HipEvents()
     stream = G.Stream()
     if args.form == "batch":
-        res = run_batch(G, stream, ev, args.steps, args.warmup)
+        res = run_batch(G, stream, ev, args.steps, args.warmup, nstreams=args.batch_streams)
         out = {"metric": "batched form: 96 uneven-grid fields (gradient + dealiased product) per "
                          "graph launch, fields/s", "value": res["f64"]["fields_per_s"],
                "unit": "fields/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

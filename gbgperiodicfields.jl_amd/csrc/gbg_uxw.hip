@@ -27,6 +27,7 @@
 // they run persistent (one workgroup per CU walking its XCD's tiles), so the
 // CU does not sit empty between one tile's workgroup and the next.
 #include <type_traits>
+#include <cstdlib>
 #include <vector>
 
 #include "fft_fast.hpp"
@@ -1241,15 +1242,14 @@ static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipS
 
 // The zero-tile writes of the last pass (the output tiles that hold no kept
 // pencil) depend on nothing the other passes compute, and the transform
-// kernel writes only the other tiles.  In FP32, when the output overlaps no
-// input, they run on the plan's second stream beside the first passes, forked
-// from s before the first pass and joined back into s after the last
-// (zero_tiles_join); 512^3 u x omega FP32: 2.92 -> 2.86 ms.  In FP64 the
-// forked form measured 0.5 % slower (the z-inverse pass leaves no room for
-// the zero writes beside it), so there, and whenever the fork is not taken,
-// the last pass launches them on s itself.  Nor is the fork taken while s is
-// being captured into a graph: the batched form's graph of many small items
-// ran 1.6x longer in FP32 with a fork and join per item.
+// kernel writes only the other tiles.  When the output overlaps no input
+// they run on the plan's second stream beside one of the first passes
+// (zero_fork_pass: K1 in FP32, K3 in FP64 with the integrating-factor
+// combination, none in FP64 otherwise), forked from s before that pass
+// and joined back into s after the last (zero_tiles_join); whenever the fork
+// is not taken the last pass launches them on s itself.  Nor is the fork
+// taken while s is being captured into a graph: the batched form's graph of
+// many small items ran 1.6x longer in FP32 with a fork and join per item.
 // A failed query or set-up call only declines the fork; its error is cleared
 // so that the next launch check does not report it.
 static bool aux_stream(gbg_plan* p, hipStream_t s) {
@@ -1297,12 +1297,29 @@ static void launch_zero_tiles(const Geo& g, const TileSet* ts, int b, int nc, cp
                        np);
 }
 
-// fork point: recorded on s before the first pass
+// The pass of u x omega beside which the zero tiles run on the second stream
+// (0: none, in stream order in the last pass).  Measured per term (min of
+// three runs of 20): 512^3 FP64 none 5.333, beside K1 5.327, K2 5.372, K3
+// 5.309, K4 5.358 ms; 1024^3 FP64 none 53.90, K1 54.17, K3 53.72 ms; 512^3
+// FP32 none 2.880, K1 2.840, K3 2.879 ms; 1024^3 FP32 all three within
+// 0.05 ms.  In FP64 the gain beside K3 is under 0.5 % for the plain term,
+// where the zero tiles only write (and it would fold their time into K3's
+// measured span), but 2.5 % for the integrating-factor step, whose zero-tile
+// kernel reads two fields and forms the exponential factor: FP64 forks them
+// beside K3 with the combination only.  GBG_ZERO_FORK_F64 / _F32 = 0..4
+// override the choice (the results are the same bits).
+template <typename T>
+static int zero_fork_pass(bool combination) {
+  const char* e = std::getenv(sizeof(T) == 8 ? "GBG_ZERO_FORK_F64" : "GBG_ZERO_FORK_F32");
+  if (e && e[0] >= '0' && e[0] <= '4' && e[1] == 0) return e[0] - '0';
+  return sizeof(T) == 8 ? (combination ? 3 : 0) : 1;
+}
+
+// fork point: recorded on s before the pass the zero tiles run beside
 template <typename T>
 static const TileSet* zero_tiles_fork(gbg_plan* p, const Geo& g, double kc, int nc, cpx<T>* out,
                                       const void* in0, size_t in0_bytes, const void* in1,
                                       size_t in1_bytes, hipStream_t s) {
-  if (sizeof(T) == 8) return nullptr;
   const size_t ob = (size_t)nc * g.M * sizeof(cpx<T>);
   if (overlaps(out, ob, in0, in0_bytes) || (in1 && overlaps(out, ob, in1, in1_bytes))) return nullptr;
   const TileSet* fork = nullptr;
@@ -1367,32 +1384,47 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   // with the combination the zero tiles write dst (the caller keeps dst off
   // u^); N^ itself may be absent
   cpx<T>* zout = ep ? ep->dst : out;
-  const TileSet* fork =
-      noalloc ? nullptr
-              : zero_tiles_fork<T>(p, g, kc, 3, zout, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
+  const int fpass = noalloc ? 0 : zero_fork_pass<T>(ep != nullptr);
+  const TileSet* fork = nullptr;
+  bool forked = false;
+  // fork before pass k is launched, start the zero writes after it
+  auto fork_at = [&](int k) {
+    if (fpass == k)
+      fork = zero_tiles_fork<T>(p, g, kc, 3, zout, uh, 3 * g.M * sizeof(cpx<T>), nullptr, 0, s);
+  };
+  auto start_at = [&](int k) {
+    if (fpass == k) forked = zero_tiles_launch<T>(p, g, fork, 3, out, ep, &tb);
+  };
   prof_begin(p, s);
+  fork_at(1);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
     const unsigned grid = launch_z_inverse<T, N>(p, g, uh, W1, tb, twz, part, s);
     hipLaunchKernelGGL(k_uxw_reduce, dim3(1), dim3(256), 0, s, part, (int)grid, P * P, p->red_out);
   });
   if ((st = check_launch("uxw K1"))) return st;
-  const bool forked = zero_tiles_launch<T>(p, g, fork, 3, out, ep, &tb);
+  start_at(1);
   prof_mark(p, s);
+  fork_at(2);
   with_len(g.ny, [&](auto L) {
     launch_y<T, decltype(L)::value, 1, 6>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("uxw K2"))) return st;
+  start_at(2);
   prof_mark(p, s);
+  fork_at(3);
   with_len(g.nx, [&](auto L) {
     launch_x<T, decltype(L)::value, false>(p->x_var, g, W2, W1, twx, T(1.0 / P), s);
   });
   if ((st = check_launch("uxw K3"))) return st;
+  start_at(3);
   prof_mark(p, s);
+  fork_at(4);
   with_len(g.ny, [&](auto L) {
     launch_y<T, decltype(L)::value, -1, 3>(g, W1, W2, tb, twy, s);
   });
   if ((st = check_launch("uxw K4"))) return st;
+  start_at(4);
   prof_mark(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
@@ -1560,7 +1592,8 @@ static gbg_status dp_fused(gbg_plan* p, const cpx<T>* ah, const cpx<T>* bh, cpx<
   const cpx<T>* twz = (const cpx<T>*)p->tw[2];
   const double P = (double)g.nx * g.ny * g.nz;
   const size_t cb1 = g.M * sizeof(cpx<T>);
-  const TileSet* fork = zero_tiles_fork<T>(p, g, kc, 1, out, ah, cb1, bh, cb1, s);
+  const TileSet* fork =
+      sizeof(T) == 4 ? zero_tiles_fork<T>(p, g, kc, 1, out, ah, cb1, bh, cb1, s) : nullptr;
   prof_begin(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;

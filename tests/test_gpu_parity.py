@@ -387,6 +387,36 @@ def test_x_pass_shapes(G, n, dtype, monkeypatch):
         assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [(64, 64, 64), (128, 64, 32)])
+def test_zero_tile_fork_points(G, n, dtype, monkeypatch):
+    """The zero tiles of the last pass beside each earlier pass on the second
+    stream (or in stream order): bitwise the same N^, also with the
+    integrating-factor combination writing over its input, and the oracle's."""
+    say(f"[zero-fork] n={n} dtype={np.dtype(dtype).name}")
+    g = ogrid(n)
+    m = O.Multipliers.build(g)
+    grid = G.Grid3D(n)
+    plans = G.FieldTransformPlans(grid, dtype)
+    u = O.synthetic_field(g, 2006, vector=True, m=m)
+    kc = O.dealias_cutoff(n[0])
+    ref = O.uxw(u, g, kc, m)[0]
+    du = G.Field(grid, "spectral", 1, dtype).upload(u)
+    var = "GBG_ZERO_FORK_F64" if dtype == np.float64 else "GBG_ZERO_FORK_F32"
+    outs = []
+    for k in "01234":
+        monkeypatch.setenv(var, k)
+        dn = G.Field(grid, "spectral", 1, dtype)
+        G.uxw_(dn, du, plans, kc)
+        dst = G.Field(grid, "spectral", 1, dtype).upload(u)
+        G.uxw_combine_(dst, du, plans, kc, a=dst, delta=0.01, coefficient=-0.002)
+        outs.append((dn.download(), dst.download()))
+        assert O.rel_l2(outs[-1][0], ref) < TOL[dtype]
+    monkeypatch.delenv(var)
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+
+
 def _fp32_mask_case():
     n, L = (64, 64, 64), (1.3, 1.3, 1.3)
     g = ogrid(n, L)
