@@ -127,6 +127,11 @@ def test_layout_pipelines(G, n, dims, dtype):
     assert np.linalg.norm(nref) > 1e-3 * np.linalg.norm(u)
     assert O.rel_l2(dn.download(), nref) < 10 * TOL[dtype]
     assert abs(E - Eref) <= 1e-5 * Eref and abs(Z - Zref) <= 1e-5 * Zref
+    # the integrating-factor combination on the same layout (composed form)
+    dst = G.Field(grid, "spectral", 1, dtype, layout=lay).upload(u)
+    G.uxw_combine_(dst, du, plans, kc, a=dst, delta=0.01, gamma=0.5, coefficient=-0.02)
+    want = O.exponential_factor(m, -0.02) * (u + 0.01 * nref) + 0.5 * nref
+    assert O.rel_l2(dst.download(), want) < 10 * TOL[dtype]
     a = O.synthetic_field(g, 4, m=m)
     da = G.Field(grid, "spectral", 0, dtype, layout=lay).upload(a)
     dp = G.Field(grid, "spectral", 0, dtype, layout=lay)
