@@ -79,6 +79,90 @@ constexpr int num_stages(int n) {
   return s;
 }
 
+// Per-stage twiddle layout (TS).  The N-entry table read at m = k N/(LS R),
+// 2m, 4m puts the butterflies k of consecutive lanes on a few LDS banks (up
+// to eight lanes on one bank).  Stage (R, LS > 1) instead owns the entries
+// ts_base(LS) + p LS + k, k in [0, LS), with w^{q_p k N/(LS R)} for q = 1, 2, 4
+// (R = 8) or q = 1 .. R - 1 (a closing radix 2 or 4): consecutive k sit in
+// consecutive entries.  The values are the N-entry table's own entries.
+constexpr int ts_npow(int r) { return r == 8 ? 3 : r - 1; }
+template <int N>
+constexpr int ts_base(int ls_want) {
+  int off = 0, ls = 1;
+  while (ls < N) {
+    const int rem = N / ls;
+    const int r = (rem % 8 == 0) ? 8 : rem;
+    if (ls == ls_want) return off;
+    if (ls > 1) off += ts_npow(r) * ls;
+    ls *= r;
+  }
+  return off;
+}
+template <int N>
+constexpr int ts_count() { return ts_base<N>(N); }
+
+// exponent e of entry i of the per-stage table (its value is tw[e]); the
+// stage loop unrolls, so ls is a constant in each test
+template <int N>
+__device__ __forceinline__ int ts_exponent(int i) {
+  constexpr int NS = num_stages(N);
+  int e = 0, off = 0, ls = 1;
+#pragma unroll
+  for (int st = 0; st < NS; ++st) {
+    const int rem = N / ls;
+    const int r = (rem % 8 == 0) ? 8 : rem;
+    if (ls > 1) {
+      const int cnt = ts_npow(r) * ls;
+      const int d = i - off;
+      if (d >= 0 && d < cnt) {
+        const int p = d / ls, k = d - p * ls;
+        e = (r == 8 ? (1 << p) : p + 1) * k * (N / (ls * r));
+      }
+      off += cnt;
+    }
+    ls *= r;
+  }
+  return e;
+}
+
+// the per-stage table from the N-entry table tw, by the NT threads of a
+// workgroup (thread tid): every load is issued before the first store
+template <typename T, int N, int NT>
+__device__ __forceinline__ void ts_fill(cpx<T>* dst, const cpx<T>* __restrict__ tw, int tid) {
+  constexpr int CNT = ts_count<N>();
+  constexpr int PER = (CNT + NT - 1) / NT;
+  cpx<T> v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int i = tid + q * NT;
+    v[q] = tw[ts_exponent<N>(i < CNT ? i : 0)];
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int i = tid + q * NT;
+    if (i < CNT) dst[i] = v[q];
+  }
+}
+
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void twiddle_ts(cpx<T>* a, int k, int ls, const cpx<T>* __restrict__ ts) {
+  if constexpr (R == 8) {
+    const cpx<T> w1 = tw_load<DIR>(ts, k), w2 = tw_load<DIR>(ts, ls + k),
+                 w4 = tw_load<DIR>(ts, 2 * ls + k);
+    const cpx<T> w3 = cmul(w2, w1);
+    a[1] = cmul(a[1], w1);
+    a[2] = cmul(a[2], w2);
+    a[3] = cmul(a[3], w3);
+    a[4] = cmul(a[4], w4);
+    a[5] = cmul(a[5], cmul(w4, w1));
+    a[6] = cmul(a[6], cmul(w4, w2));
+    a[7] = cmul(a[7], cmul(w4, w3));
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw_load<DIR>(ts, (r - 1) * ls + k));
+  }
+}
+
 // LDS slot of pencil element e: PD = 0 stores e at e; PD = 1 leaves one
 // empty slot after every 8 (slot e + e/8, a pencil spans N + N/8 slots).  With
 // the pencil strides of Cfg this spreads the stride-8 writes of the first
@@ -100,7 +184,8 @@ constexpr int slot_step(int c) {
 // suffices -- a buffer is rewritten only two exchanges after it was read.
 // ONE (p0 == p1, one buffer where two do not fit in LDS): a block barrier also
 // separates each stage's reads from its writes.
-template <typename T, int N, int E, int LS, int DIR, int SYNC, int W, int PD, bool ONE>
+// TS: tw is the per-stage table (ts_count<N>() entries) instead of the N-entry one.
+template <typename T, int N, int E, int LS, int DIR, int SYNC, int W, int PD, bool ONE, bool TS>
 __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                        const cpx<T>* __restrict__ tw) {
   constexpr int R = StageR<N, LS>::R;
@@ -134,7 +219,11 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
   for (int b = 0; b < NB; ++b) {
     if constexpr (!FIRST) {
       const int j = t + TP * b;
-      twiddle<T, R, DIR>(a[b], j % LS, N / (LS * R), tw);
+      constexpr int TB = ts_base<N>(LS);   // a constant, not a loop in the kernel
+      if constexpr (TS)
+        twiddle_ts<T, R, DIR>(a[b], j % LS, LS, tw + TB);
+      else
+        twiddle<T, R, DIR>(a[b], j % LS, N / (LS * R), tw);
     }
     Dft<R, DIR, T>::run(a[b], tw, N);
   }
@@ -154,18 +243,18 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
       for (int r = 0; r < R; ++r) o[r * (LS == 1 ? 1 : slot_step<PD>(LS))] = a[b][r];
     }
     lds_sync<SYNC>();
-    stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD, ONE>(v, p0, p1, t, tw);
+    stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD, ONE, TS>(v, p0, p1, t, tw);
   }
 }
 
 // In-register FFT of one pencil: v[m] = x[t + TP m] in and out.  p0, p1 are
 // this pencil's two LDS regions (slot<PD> layout, at least N elements; the
 // same region with ONE).  On return p0 may be written at once (its last read
-// was followed by a barrier); p1 may not.
-template <typename T, int N, int E, int DIR, int SYNC, int PD, bool ONE = false>
+// was followed by a barrier); p1 may not.  TS: tw is the per-stage table.
+template <typename T, int N, int E, int DIR, int SYNC, int PD, bool ONE = false, bool TS = false>
 __device__ __forceinline__ void fft(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, int t,
                                     const cpx<T>* __restrict__ tw) {
-  stages<T, N, E, 1, DIR, SYNC, 0, PD, ONE>(v, p0, p1, t, tw);
+  stages<T, N, E, 1, DIR, SYNC, 0, PD, ONE, TS>(v, p0, p1, t, tw);
   // with an odd number of exchanges the last one read p0; with one buffer
   // the last exchange always read it
   if constexpr (ONE || (num_stages(N) - 1) % 2 == 1) lds_sync<SYNC>();

@@ -208,6 +208,13 @@ static void destroy_plan(gbg_plan* p) {
   if (p->aux_fork) (void)hipEventDestroy(p->aux_fork);
   if (p->aux_join) (void)hipEventDestroy(p->aux_join);
   if (p->aux) (void)hipStreamDestroy(p->aux);
+  for (int i = 0; i < kSlabStreams; ++i) {
+    if (p->slab_join[i]) (void)hipEventDestroy(p->slab_join[i]);
+    if (p->slab_st[i]) (void)hipStreamDestroy(p->slab_st[i]);
+  }
+  for (int i = 0; i < 3 * kSlabStreams; ++i)
+    if (p->slab_ev[i]) (void)hipEventDestroy(p->slab_ev[i]);
+  if (p->slab_fork) (void)hipEventDestroy(p->slab_fork);
   delete p;
 }
 

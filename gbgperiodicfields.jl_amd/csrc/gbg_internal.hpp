@@ -89,6 +89,7 @@ struct Tables {
 // ---------------------------------------------------------------------------
 constexpr int kMaxEvents = 16;
 constexpr int kRedBlocks = 1024;   // fixed reduction grid: deterministic sums
+constexpr int kSlabStreams = 4;    // at most this many slabs of the z-slab chain in flight
 
 // Output tiles (B consecutive (ky, kx) pencils) of the fused pipeline's last
 // pass for one cutoff: dev[0, nd) lists the tiles that hold a kept (kx, ky)
@@ -150,6 +151,13 @@ struct gbg_plan {
   // on first use outside a graph capture)
   hipStream_t aux;
   hipEvent_t aux_fork, aux_join;
+  // further streams of the fused pipelines' z-slab chain (gbg_uxw.hip), its
+  // fork event and one join event per stream (created on first use outside a
+  // graph capture)
+  hipStream_t slab_st[gbg::kSlabStreams];
+  hipEvent_t slab_fork;
+  hipEvent_t slab_join[gbg::kSlabStreams];
+  hipEvent_t slab_ev[3 * gbg::kSlabStreams];
   // profiling events
   int profiling;
   int nev;

@@ -572,12 +572,6 @@ struct XStage {
   static constexpr int TPE = Cfg<T, N>::XTP * EPL;             // entries per copy per unit
   static constexpr int CAP = CH * TPE;                         // staged entries per row
   static constexpr int WAVE_ELEMS = 2 * CH * 64 * EPL;         // staging elements per wave
-  static constexpr int ALL8 = (fast::num_stages(N) > 0) && ([] {
-    int n = N;
-    while (n % 8 == 0) n /= 8;
-    return n == 1;
-  })();
-  static constexpr int TWN = ALL8 ? N / 2 : N;                 // twiddle entries the stages read
 };
 
 template <typename T, int N, bool STG, bool PROD, int V>
@@ -594,10 +588,12 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
   constexpr int NW = C::XNT / 64;
   __shared__ cpx<T> lds[NBUF * C::U * LD];
   __shared__ T held[S ? C::U * E * TP : 1];
-  __shared__ cpx<T> twl[S ? X::TWN : 1];
+  // with S the stages read the per-stage twiddle table (bank-conflict free
+  // for the consecutive butterflies of a wave's lanes)
+  __shared__ cpx<T> twl[S ? fast::ts_count<N>() : 1];
   __shared__ cpx<T> stg[STG ? NW * X::WAVE_ELEMS : 1];
   if (S) {
-    for (int i = threadIdx.x; i < X::TWN; i += C::XNT) twl[i] = tw[i];
+    fast::ts_fill<T, N, C::XNT>(twl, tw, threadIdx.x);
     __syncthreads();
   }
   auto twp = [&]() { return S ? (const cpx<T>*)twl : fresh<sizeof(T) == 8>(tw); };
@@ -663,7 +659,7 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
       issue(ro1, 0);
       fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
     }
-    fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
+    fast::fft<T, N, E, 1, S, C::XPD, false, S != 0>(z, pen, pen1, t, twp());
 #pragma unroll
     for (int m = 0; m < E; ++m) P0[m] = (z[m].x * scale) * (z[m].y * scale);
     {
@@ -671,10 +667,10 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
       fetch(a, b, ro1, ok1, 0);
       fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
     }
-    fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
+    fast::fft<T, N, E, 1, S, C::XPD, false, S != 0>(z, pen, pen1, t, twp());
 #pragma unroll
     for (int m = 0; m < E; ++m) z[m] = mk<T>(P0[m], (z[m].x * scale) * (z[m].y * scale));
-    fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, twp());
+    fast::fft<T, N, E, -1, S, C::XPD, false, S != 0>(z, pen, pen1, t, twp());
     fast::r2c_pair_store<T, N, E>(z, pen, w3 + ro0, w3 + ro1, ok0, ok1, t, g);
     return;
   }
@@ -711,7 +707,7 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
           issue(ro1, 0);
         fast::c2r_pair_combine<T, N, E>(z, a, b, t, g);
       }
-      fast::fft<T, N, E, 1, S, C::XPD>(z, pen, pen1, t, twp());
+      fast::fft<T, N, E, 1, S, C::XPD, false, S != 0>(z, pen, pen1, t, twp());
       if (q == 0) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -731,7 +727,7 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
           const T u3 = z[m].x * scale, w3v = z[m].y * scale;
           z[m] = mk<T>(U2[m] * w3v - u3 * W2[m], u3 * W1[m] - U1[m] * w3v);
         }
-        fast::fft<T, N, E, -1, S, C::XPD>(z, pen, pen1, t, twp());
+        fast::fft<T, N, E, -1, S, C::XPD, false, S != 0>(z, pen, pen1, t, twp());
         if (EARLY && h == 0) {
           fetch(na, nb, ro1, ok1, 0);
           issue(ro1, 1);
@@ -759,7 +755,7 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
   cpx<T> v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = mk<T>(S ? hold[m * TP] : n3a[m], N3[m]);
-  fast::fft<T, N, E, -1, S, C::XPD>(v, pen, pen1, t, twp());
+  fast::fft<T, N, E, -1, S, C::XPD, false, S != 0>(v, pen, pen1, t, twp());
   fast::r2c_pair_store<T, N, E>(v, pen, w3 + 2 * g.s2 + ro0, w3 + 2 * g.s2 + ro1, ok0, ok1, t, g);
 }
 
@@ -1360,6 +1356,181 @@ static gbg_status zero_tiles_join(gbg_plan* p, hipStream_t s) {
   return check_hip(hipStreamWaitEvent(s, p->aux_join, 0), "join of the zero-tile stream");
 }
 
+// ---- the z-slab chain of the middle passes ------------------------------------
+// K2, K3 and K4 (P2, P3, P4 of the scalar product) each work inside z planes:
+// a y pencil (z, kx) and an x row (z, y) lie in one plane.  They can therefore
+// run slab by slab, Z planes at a time, with a slab's full-y fields (NF after
+// the y-inverse, NO after the x pass) in a ring of R slots of (NF + NO) Z Ny nhp
+// entries instead of two full-size buffers.  A slot is read back soon after it
+// is written, while the device's memory-side last-level cache (256 MB on the
+// MI355X) still holds it, so the schedule's largest intermediates mostly stay
+// off HBM.  The y-forward of slab j writes its compact output into the first
+// NO fields of the compact buffer at slab j's planes, whose input the
+// y-inverse of slab j has already read (both in stream order); the last pass
+// reads it there.  Slab j runs on stream j mod R (s itself for j mod R = 0),
+// forked from s after the first pass and joined back before the last; under a
+// graph capture, for a caller that must not create resources, or when a
+// stream cannot be made, every slab runs on s in order.
+struct SlabCfg {
+  int z;   // planes per slab (0: no slab chain, the two full-size buffers)
+  int r;   // ring slots, the slabs in flight
+  int pp;  // 1: one stream per pass, slabs flowing through them under events
+};
+
+static SlabCfg slab_cfg(const gbg_plan* p, const Geo& g) {
+  (void)p;
+  const char* e = std::getenv("GBG_UXW_SLAB");
+  if (e) {
+    char* end = nullptr;
+    const long z = std::strtol(e, &end, 10);
+    if (z > 0 && end && *end == ',') {
+      const long r = std::strtol(end + 1, &end, 10);
+      long pp = 0;
+      if (end && *end == ',') pp = std::strtol(end + 1, &end, 10);
+      if (r >= 1 && r <= kSlabStreams && (pp == 0 || pp == 1) && end && *end == 0)
+        return SlabCfg{(int)(z < g.nz ? z : g.nz), (int)r, (int)pp};
+    }
+    return SlabCfg{0, 0, 0};
+  }
+  return SlabCfg{0, 0, 0};
+}
+
+// the chain's extra streams and events: nst streams besides s and ne events
+// of slab_ev (false: run on s)
+static bool slab_streams(gbg_plan* p, hipStream_t s, int nst, int ne) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (cs != hipStreamCaptureStatusNone) return false;
+  if (!p->slab_fork && hipEventCreateWithFlags(&p->slab_fork, hipEventDisableTiming) != hipSuccess) {
+    p->slab_fork = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  for (int i = 0; i < ne; ++i) {
+    if (!p->slab_ev[i] && hipEventCreateWithFlags(&p->slab_ev[i], hipEventDisableTiming) != hipSuccess) {
+      p->slab_ev[i] = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+  }
+  for (int i = 0; i < nst; ++i) {
+    if (!p->slab_st[i] && hipStreamCreateWithFlags(&p->slab_st[i], hipStreamNonBlocking) != hipSuccess) {
+      p->slab_st[i] = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    if (!p->slab_join[i] &&
+        hipEventCreateWithFlags(&p->slab_join[i], hipEventDisableTiming) != hipSuccess) {
+      p->slab_join[i] = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+  }
+  return true;
+}
+
+// Entries of one ring slot (NF + NO full-y fields of Z planes)
+static long long slab_slot(const Geo& g, const SlabCfg& sc, int nf, int no) {
+  return (long long)(nf + no) * sc.z * g.ny * g.nhp;
+}
+
+// The middle passes slab by slab: wc the compact buffer (NF fields in, the
+// first NO fields out, field stride g.s1), ring the R slots.
+template <typename T, int NF, int NO>
+static gbg_status slab_chain(gbg_plan* p, const Geo& g, const SlabCfg& sc, cpx<T>* wc, cpx<T>* ring,
+                             const Tables& tb, const cpx<T>* twx, const cpx<T>* twy, T scale,
+                             bool noalloc, hipStream_t s) {
+  constexpr bool PROD = NF == 2;
+  const long long slot = slab_slot(g, sc, NF, NO);
+  const int nsl = (g.nz + sc.z - 1) / sc.z;
+  const int r = sc.r < nsl ? sc.r : nsl;
+  gbg_status st = GBG_OK;
+  if (sc.pp && !noalloc && slab_streams(p, s, 2, 3 * r)) {
+    // y-inverse on slab_st[0], x pass on slab_st[1], y-forward on s; events
+    // ev[0 .. r) y-inverse done, ev[r .. 2r) x pass done, ev[2r .. 3r) y-forward
+    // done, slot k = j mod r.  The y-inverse of slab j rewrites the slot's
+    // input fields, read by the x pass of slab j - r; the x pass of slab j
+    // rewrites its output fields, read by the y-forward of slab j - r.
+    hipStream_t q2 = p->slab_st[0], q3 = p->slab_st[1];
+    hipEvent_t* e2 = p->slab_ev;
+    hipEvent_t* e3 = p->slab_ev + r;
+    hipEvent_t* e4 = p->slab_ev + 2 * r;
+    auto ok = [&](hipError_t e) { return !(st = check_hip(e, "slab chain")); };
+    if (!ok(hipEventRecord(p->slab_fork, s)) || !ok(hipStreamWaitEvent(q2, p->slab_fork, 0)) ||
+        !ok(hipStreamWaitEvent(q3, p->slab_fork, 0)))
+      return st;
+    for (int j = 0; j < nsl; ++j) {
+      const int k = j % r;
+      const int z0 = j * sc.z, zn = sc.z < g.nz - z0 ? sc.z : g.nz - z0;
+      Geo gs = g;
+      gs.nz = zn;
+      gs.s2 = (long long)zn * g.ny * g.nhp;
+      cpx<T>* fin = ring + k * slot;
+      cpx<T>* fout = fin + NF * gs.s2;
+      cpx<T>* cz = wc + (long long)z0 * g.nyc * g.nhp;
+      if (j >= r && !ok(hipStreamWaitEvent(q2, e3[k], 0))) return st;
+      with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, 1, NF>(gs, cz, fin, tb, twy, q2); });
+      if (!ok(hipEventRecord(e2[k], q2)) || !ok(hipStreamWaitEvent(q3, e2[k], 0))) return st;
+      if (j >= r && !ok(hipStreamWaitEvent(q3, e4[k], 0))) return st;
+      with_len(g.nx, [&](auto L) {
+        launch_x<T, decltype(L)::value, PROD>(p->x_var, gs, fin, fout, twx, scale, q3);
+      });
+      if (!ok(hipEventRecord(e3[k], q3)) || !ok(hipStreamWaitEvent(s, e3[k], 0))) return st;
+      with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, -1, NO>(gs, fout, cz, tb, twy, s); });
+      if (!ok(hipEventRecord(e4[k], s))) return st;
+      if ((st = check_launch("slab chain"))) return st;
+    }
+    return GBG_OK;
+  }
+  bool multi = r > 1 && !noalloc && slab_streams(p, s, r - 1, 0);
+  if (multi) {
+    if ((st = check_hip(hipEventRecord(p->slab_fork, s), "slab chain fork"))) return st;
+    for (int i = 0; i + 1 < r; ++i)
+      if ((st = check_hip(hipStreamWaitEvent(p->slab_st[i], p->slab_fork, 0), "slab chain fork")))
+        return st;
+  }
+  for (int j = 0; j < nsl; ++j) {
+    const int k = j % r;
+    const hipStream_t q = (multi && k > 0) ? p->slab_st[k - 1] : s;
+    const int z0 = j * sc.z, zn = sc.z < g.nz - z0 ? sc.z : g.nz - z0;
+    Geo gs = g;
+    gs.nz = zn;
+    gs.s2 = (long long)zn * g.ny * g.nhp;
+    cpx<T>* fin = ring + k * slot;      // NF full-y fields of the slab
+    cpx<T>* fout = fin + NF * gs.s2;    // NO full-y fields of the slab
+    cpx<T>* cz = wc + (long long)z0 * g.nyc * g.nhp;
+    with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, 1, NF>(gs, cz, fin, tb, twy, q); });
+    with_len(g.nx, [&](auto L) {
+      launch_x<T, decltype(L)::value, PROD>(p->x_var, gs, fin, fout, twx, scale, q);
+    });
+    with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, -1, NO>(gs, fout, cz, tb, twy, q); });
+    if ((st = check_launch("slab chain"))) return st;
+  }
+  if (multi) {
+    for (int i = 0; i + 1 < r; ++i) {
+      if ((st = check_hip(hipEventRecord(p->slab_join[i], p->slab_st[i]), "slab chain join")))
+        return st;
+      if ((st = check_hip(hipStreamWaitEvent(s, p->slab_join[i], 0), "slab chain join"))) return st;
+    }
+  }
+  return GBG_OK;
+}
+
+// bytes of u x omega's two intermediate buffers (element size cb)
+static void uxw_buffers(const Geo& g, const SlabCfg& sc, size_t cb, size_t* b1, size_t* b2) {
+  if (sc.z > 0) {
+    *b1 = (size_t)6 * g.s1 * cb;
+    const int nsl = (g.nz + sc.z - 1) / sc.z;
+    *b2 = (size_t)(sc.r < nsl ? sc.r : nsl) * slab_slot(g, sc, 6, 3) * cb;
+    return;
+  }
+  *b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cb;
+  *b2 = (size_t)6 * g.s2 * cb;
+}
+
 // base: uxw_work_bytes of work area.  With noalloc the pass schedule creates
 // nothing (no tile set, no second stream): the last pass uses a tile set only
 // when the plan already caches one for this cutoff.
@@ -1369,9 +1540,12 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
                             const Epi<T>* ep = nullptr) {
   const size_t cb = sizeof(cpx<T>);
   // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
-  // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields
-  const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cb;
-  const size_t b2 = (size_t)6 * g.s2 * cb;
+  // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields.
+  // With the z-slab chain: W1 the compact fields (K4 writes its 3 into the
+  // first three), W2 the ring of slots.
+  const SlabCfg sc = slab_cfg(p, g);
+  size_t b1, b2;
+  uxw_buffers(g, sc, cb, &b1, &b2);
   gbg_status st = GBG_OK;
   cpx<T>* W1 = (cpx<T>*)base;
   cpx<T>* W2 = (cpx<T>*)(base + b1);
@@ -1405,26 +1579,39 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
   if ((st = check_launch("uxw K1"))) return st;
   start_at(1);
   prof_mark(p, s);
-  fork_at(2);
-  with_len(g.ny, [&](auto L) {
-    launch_y<T, decltype(L)::value, 1, 6>(g, W1, W2, tb, twy, s);
-  });
-  if ((st = check_launch("uxw K2"))) return st;
-  start_at(2);
-  prof_mark(p, s);
-  fork_at(3);
-  with_len(g.nx, [&](auto L) {
-    launch_x<T, decltype(L)::value, false>(p->x_var, g, W2, W1, twx, T(1.0 / P), s);
-  });
-  if ((st = check_launch("uxw K3"))) return st;
-  start_at(3);
-  prof_mark(p, s);
-  fork_at(4);
-  with_len(g.ny, [&](auto L) {
-    launch_y<T, decltype(L)::value, -1, 3>(g, W1, W2, tb, twy, s);
-  });
-  if ((st = check_launch("uxw K4"))) return st;
-  start_at(4);
+  // K4's compact output: W2 with the two full-size buffers, W1 with the chain
+  cpx<T>* W4 = W2;
+  if (sc.z > 0) {
+    // one span for K2..K4 (a fork point in 2..4 forks before the chain)
+    if (fpass >= 2 && fpass <= 4) {
+      fork_at(fpass);
+      start_at(fpass);
+    }
+    if ((st = slab_chain<T, 6, 3>(p, g, sc, W1, W2, tb, twx, twy, T(1.0 / P), noalloc, s)))
+      return st;
+    W4 = W1;
+  } else {
+    fork_at(2);
+    with_len(g.ny, [&](auto L) {
+      launch_y<T, decltype(L)::value, 1, 6>(g, W1, W2, tb, twy, s);
+    });
+    if ((st = check_launch("uxw K2"))) return st;
+    start_at(2);
+    prof_mark(p, s);
+    fork_at(3);
+    with_len(g.nx, [&](auto L) {
+      launch_x<T, decltype(L)::value, false>(p->x_var, g, W2, W1, twx, T(1.0 / P), s);
+    });
+    if ((st = check_launch("uxw K3"))) return st;
+    start_at(3);
+    prof_mark(p, s);
+    fork_at(4);
+    with_len(g.ny, [&](auto L) {
+      launch_y<T, decltype(L)::value, -1, 3>(g, W1, W2, tb, twy, s);
+    });
+    if ((st = check_launch("uxw K4"))) return st;
+    start_at(4);
+  }
   prof_mark(p, s);
   with_len(g.nz, [&](auto L) {
     constexpr int N = decltype(L)::value;
@@ -1433,12 +1620,12 @@ static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Ge
     const TileSet* ts = tile_set(p, g, C::B, kc, s, !noalloc);
     if (!ts) {
       const long long nt = (plane + C::B - 1) / C::B;
-      launch_z_forward<T, N, 3>(p, g, nt, W2, out, tb, twz, nullptr, s, ep);
+      launch_z_forward<T, N, 3>(p, g, nt, W4, out, tb, twz, nullptr, s, ep);
       return;
     }
     if (!forked && ts->nd < ts->nt) launch_zero_tiles<T>(g, ts, C::B, 3, out, s, ep, &tb);
     if (ts->nd > 0)
-      launch_z_forward<T, N, 3>(p, g, ts->nd, W2, out, tb, twz, ts->dev, s, ep);
+      launch_z_forward<T, N, 3>(p, g, ts->nd, W4, out, tb, twz, ts->dev, s, ep);
   });
   if ((st = check_launch("uxw K5"))) return st;
   if (forked && (st = zero_tiles_join(p, s))) return st;
@@ -1458,8 +1645,8 @@ size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags) {
   const Geo g = make_geo(p, kc);
   // W1, W2 and the K1 partials (2 doubles per launched workgroup, at most
   // one per tile plus the XCD padding)
-  const size_t b1 = (size_t)(6 * g.s1 > 3 * g.s2 ? 6 * g.s1 : 3 * g.s2) * cs;
-  const size_t b2 = (size_t)6 * g.s2 * cs;
+  size_t b1, b2;
+  uxw_buffers(g, slab_cfg(p, g), cs, &b1, &b2);
   return b1 + b2 + sizeof(double) * 2 * ((size_t)g.nyc * g.nhp + fast::kXcd);
 }
 
