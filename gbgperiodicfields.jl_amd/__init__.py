@@ -666,18 +666,24 @@ def reality_projection_(dst: Field, *args, stream=None) -> Field:
 
 # ---- fused pipelines ---------------------------------------------------------------
 
+def _uxw_flags(fused: bool, slabs: bool) -> int:
+    return (0 if fused else _lib.GBG_UXW_UNFUSED) | (_lib.GBG_UXW_SLABS if slabs else 0)
+
+
 def uxw_(n_hat: Field, u_hat: Field, plans: FieldTransformPlans, kc: float | None = None,
-         fused: bool = True, reductions: bool = True, stream=None):
+         fused: bool = True, reductions: bool = True, slabs: bool = False, stream=None):
     """The dealiased rotational nonlinear term ``N^ = Q M P F(u x w)`` of
     ``examples/HIST_CPU.jl:229-247`` in one call.  Returns ``(E, Z)`` (FP64)
-    when ``reductions`` is true, else ``None`` (no synchronisation)."""
+    when ``reductions`` is true, else ``None`` (no synchronisation).
+    ``slabs``: the middle passes z-slab by z-slab (same bits, smaller work
+    area)."""
     _need(u_hat, "spectral", 1)
     _need(n_hat, "spectral", 1)
     _with_plan(u_hat, plans)
     _same_domain(n_hat, u_hat)
     if kc is None:
         kc = dealias_cutoff(plans.grid.grid_size[0], plans.grid.domain_size[0])
-    flags = 0 if fused else _lib.GBG_UXW_UNFUSED
+    flags = _uxw_flags(fused, slabs)
     if reductions:
         E, Z = C.c_double(0.0), C.c_double(0.0)
         check(load().gbg_uxw(plans.handle, u_hat.data, n_hat.data, float(kc), C.byref(E), C.byref(Z),
@@ -692,7 +698,7 @@ def uxw_combine_(dst: Field, u_hat: Field, plans: FieldTransformPlans, kc: float
                  n_hat: Field | None = None, a: Field | None = None, alpha: float = 1.0,
                  b: Field | None = None, beta: float = 0.0, delta: float = 0.0, gamma: float = 0.0,
                  coefficient: float = 0.0, fused: bool = True, reductions: bool = True,
-                 stream=None):
+                 slabs: bool = False, stream=None):
     """u x omega with the integrating-factor step of ``step!``
     (``examples/HIST_CPU.jl:267-282``) folded into its last pass:
     ``dst = exp(c|k|^2) (alpha a + beta b + delta N^) + gamma N^``, and N^ into
@@ -712,7 +718,7 @@ def uxw_combine_(dst: Field, u_hat: Field, plans: FieldTransformPlans, kc: float
     cmb = _lib.GbgUxwCombination(dst.data, None if a is None else a.data, float(alpha),
                                  None if b is None else b.data, float(beta), float(delta),
                                  float(gamma), float(coefficient))
-    flags = 0 if fused else _lib.GBG_UXW_UNFUSED
+    flags = _uxw_flags(fused, slabs)
     E, Z = C.c_double(0.0), C.c_double(0.0)
     check(load().gbg_uxw_combine(plans.handle, u_hat.data, None if n_hat is None else n_hat.data,
                                  float(kc), C.byref(cmb), C.byref(E) if reductions else None,

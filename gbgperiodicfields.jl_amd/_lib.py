@@ -17,6 +17,7 @@ STATUS_NAMES = {0: "GBG_OK", 1: "GBG_E_ARG", 2: "GBG_E_DIM", 3: "GBG_E_UNSUPPORT
                 4: "GBG_E_ALIAS", 5: "GBG_E_OOM", 6: "GBG_E_DEVICE"}
 
 GBG_UXW_UNFUSED = 1
+GBG_UXW_SLABS = 2
 
 class GbgJob(C.Structure):
     """struct gbg_job of include/gbg.h."""

@@ -212,8 +212,6 @@ static void destroy_plan(gbg_plan* p) {
     if (p->slab_join[i]) (void)hipEventDestroy(p->slab_join[i]);
     if (p->slab_st[i]) (void)hipStreamDestroy(p->slab_st[i]);
   }
-  for (int i = 0; i < 3 * kSlabStreams; ++i)
-    if (p->slab_ev[i]) (void)hipEventDestroy(p->slab_ev[i]);
   if (p->slab_fork) (void)hipEventDestroy(p->slab_fork);
   delete p;
 }
@@ -278,6 +276,20 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, const
     p->x_var = n[0] >= 1024 ? 1 : 0;   // the x pass's default shape (gbg_uxw.hip)
     const char* xs = std::getenv("GBG_X_SHAPE");
     if (xs && (xs[0] == '0' || xs[0] == '1') && xs[1] == 0) p->x_var = xs[0] - '0';
+    // the z-slab chain of the fused pipelines (gbg_uxw.hip): Z,R or 0 (off)
+    const char* sl = std::getenv("GBG_UXW_SLAB");
+    if (sl) {
+      p->slab_set = 1;
+      char* end = nullptr;
+      const long z = std::strtol(sl, &end, 10);
+      if (z > 0 && end && *end == ',') {
+        const long r = std::strtol(end + 1, &end, 10);
+        if (r >= 1 && r <= kSlabStreams && end && *end == 0) {
+          p->slab_z = (int)(z < n[2] ? z : n[2]);
+          p->slab_r = (int)r;
+        }
+      }
+    }
   }
   for (int a = 0; a < 3; ++a) {
     p->use_fast[a] = 1;

@@ -157,7 +157,9 @@ struct gbg_plan {
   hipStream_t slab_st[gbg::kSlabStreams];
   hipEvent_t slab_fork;
   hipEvent_t slab_join[gbg::kSlabStreams];
-  hipEvent_t slab_ev[3 * gbg::kSlabStreams];
+  // GBG_UXW_SLAB=Z,R (or 0) at creation: slab_set = 1 and the chain's planes
+  // per slab and slabs in flight (slab_z = 0: off)
+  int slab_set, slab_z, slab_r;
   // profiling events
   int profiling;
   int nev;

@@ -1368,36 +1368,40 @@ static gbg_status zero_tiles_join(gbg_plan* p, hipStream_t s) {
 // NO fields of the compact buffer at slab j's planes, whose input the
 // y-inverse of slab j has already read (both in stream order); the last pass
 // reads it there.  Slab j runs on stream j mod R (s itself for j mod R = 0),
-// forked from s after the first pass and joined back before the last; under a
-// graph capture, for a caller that must not create resources, or when a
+// forked from s after the first pass and joined back before the last; for a
+// caller that must not create resources, under a graph capture, or when a
 // stream cannot be made, every slab runs on s in order.
 struct SlabCfg {
   int z;   // planes per slab (0: no slab chain, the two full-size buffers)
   int r;   // ring slots, the slabs in flight
-  int pp;  // 1: one stream per pass, slabs flowing through them under events
 };
 
-static SlabCfg slab_cfg(const gbg_plan* p, const Geo& g) {
-  (void)p;
-  const char* e = std::getenv("GBG_UXW_SLAB");
-  if (e) {
-    char* end = nullptr;
-    const long z = std::strtol(e, &end, 10);
-    if (z > 0 && end && *end == ',') {
-      const long r = std::strtol(end + 1, &end, 10);
-      long pp = 0;
-      if (end && *end == ',') pp = std::strtol(end + 1, &end, 10);
-      if (r >= 1 && r <= kSlabStreams && (pp == 0 || pp == 1) && end && *end == 0)
-        return SlabCfg{(int)(z < g.nz ? z : g.nz), (int)r, (int)pp};
-    }
-    return SlabCfg{0, 0, 0};
-  }
-  return SlabCfg{0, 0, 0};
+// The chain runs when the caller asks for it (GBG_UXW_SLABS): Z planes per
+// slab such that a ring slot holds at most kSlabSlotBytes (two slots in
+// flight stay within the 256 MB cache), two slots.  Measured on the MI355X
+// (u x omega, medians of interleaved runs): 512^3 FP64 5.26 -> 5.21 ms (8
+// planes per slab), 1024^3 FP64 54.06 -> 53.58 ms (2 planes), 512^3 FP32
+// 2.86 -> 2.82 ms (16 planes), 1024^3 FP32 30.0 -> 30.2 ms (8 planes; 4 and
+// 2 planes slower still); on smaller grids the short per-slab launches cannot
+// fill the device.  Three slots in flight, and one stream per pass with the
+// slabs flowing through them under events, measured slower.  What the chain
+// saves is memory: the six full-y fields (6 Nz Ny nhp entries, 4.4 GB at
+// 512^3 FP64, 35 GB at 1024^3) become two slots.  GBG_UXW_SLAB=Z,R at plan
+// creation sets the planes per slab and the slots for such calls (tests and
+// tuning); the results are the same bits for every setting.
+constexpr size_t kSlabSlotBytes = (size_t)104 << 20;
+
+static SlabCfg slab_cfg(const gbg_plan* p, const Geo& g, unsigned flags) {
+  if (!(flags & GBG_UXW_SLABS)) return SlabCfg{0, 0};
+  if (p->slab_set) return SlabCfg{p->slab_z, p->slab_r};
+  const size_t plane = (size_t)9 * g.ny * g.nhp * 2 * p->elem;
+  const size_t z = kSlabSlotBytes / plane;
+  return SlabCfg{z < 1 ? 1 : (z < (size_t)g.nz ? (int)z : g.nz), 2};
 }
 
-// the chain's extra streams and events: nst streams besides s and ne events
-// of slab_ev (false: run on s)
-static bool slab_streams(gbg_plan* p, hipStream_t s, int nst, int ne) {
+// the chain's nst streams besides s and their events (false: run on s, as
+// while s is being captured into a graph)
+static bool slab_streams(gbg_plan* p, hipStream_t s, int nst) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
     (void)hipGetLastError();
@@ -1408,13 +1412,6 @@ static bool slab_streams(gbg_plan* p, hipStream_t s, int nst, int ne) {
     p->slab_fork = nullptr;
     (void)hipGetLastError();
     return false;
-  }
-  for (int i = 0; i < ne; ++i) {
-    if (!p->slab_ev[i] && hipEventCreateWithFlags(&p->slab_ev[i], hipEventDisableTiming) != hipSuccess) {
-      p->slab_ev[i] = nullptr;
-      (void)hipGetLastError();
-      return false;
-    }
   }
   for (int i = 0; i < nst; ++i) {
     if (!p->slab_st[i] && hipStreamCreateWithFlags(&p->slab_st[i], hipStreamNonBlocking) != hipSuccess) {
@@ -1448,44 +1445,7 @@ static gbg_status slab_chain(gbg_plan* p, const Geo& g, const SlabCfg& sc, cpx<T
   const int nsl = (g.nz + sc.z - 1) / sc.z;
   const int r = sc.r < nsl ? sc.r : nsl;
   gbg_status st = GBG_OK;
-  if (sc.pp && !noalloc && slab_streams(p, s, 2, 3 * r)) {
-    // y-inverse on slab_st[0], x pass on slab_st[1], y-forward on s; events
-    // ev[0 .. r) y-inverse done, ev[r .. 2r) x pass done, ev[2r .. 3r) y-forward
-    // done, slot k = j mod r.  The y-inverse of slab j rewrites the slot's
-    // input fields, read by the x pass of slab j - r; the x pass of slab j
-    // rewrites its output fields, read by the y-forward of slab j - r.
-    hipStream_t q2 = p->slab_st[0], q3 = p->slab_st[1];
-    hipEvent_t* e2 = p->slab_ev;
-    hipEvent_t* e3 = p->slab_ev + r;
-    hipEvent_t* e4 = p->slab_ev + 2 * r;
-    auto ok = [&](hipError_t e) { return !(st = check_hip(e, "slab chain")); };
-    if (!ok(hipEventRecord(p->slab_fork, s)) || !ok(hipStreamWaitEvent(q2, p->slab_fork, 0)) ||
-        !ok(hipStreamWaitEvent(q3, p->slab_fork, 0)))
-      return st;
-    for (int j = 0; j < nsl; ++j) {
-      const int k = j % r;
-      const int z0 = j * sc.z, zn = sc.z < g.nz - z0 ? sc.z : g.nz - z0;
-      Geo gs = g;
-      gs.nz = zn;
-      gs.s2 = (long long)zn * g.ny * g.nhp;
-      cpx<T>* fin = ring + k * slot;
-      cpx<T>* fout = fin + NF * gs.s2;
-      cpx<T>* cz = wc + (long long)z0 * g.nyc * g.nhp;
-      if (j >= r && !ok(hipStreamWaitEvent(q2, e3[k], 0))) return st;
-      with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, 1, NF>(gs, cz, fin, tb, twy, q2); });
-      if (!ok(hipEventRecord(e2[k], q2)) || !ok(hipStreamWaitEvent(q3, e2[k], 0))) return st;
-      if (j >= r && !ok(hipStreamWaitEvent(q3, e4[k], 0))) return st;
-      with_len(g.nx, [&](auto L) {
-        launch_x<T, decltype(L)::value, PROD>(p->x_var, gs, fin, fout, twx, scale, q3);
-      });
-      if (!ok(hipEventRecord(e3[k], q3)) || !ok(hipStreamWaitEvent(s, e3[k], 0))) return st;
-      with_len(g.ny, [&](auto L) { launch_y<T, decltype(L)::value, -1, NO>(gs, fout, cz, tb, twy, s); });
-      if (!ok(hipEventRecord(e4[k], s))) return st;
-      if ((st = check_launch("slab chain"))) return st;
-    }
-    return GBG_OK;
-  }
-  bool multi = r > 1 && !noalloc && slab_streams(p, s, r - 1, 0);
+  bool multi = r > 1 && !noalloc && slab_streams(p, s, r - 1);
   if (multi) {
     if ((st = check_hip(hipEventRecord(p->slab_fork, s), "slab chain fork"))) return st;
     for (int i = 0; i + 1 < r; ++i)
@@ -1536,14 +1496,13 @@ static void uxw_buffers(const Geo& g, const SlabCfg& sc, size_t cb, size_t* b1, 
 // when the plan already caches one for this cutoff.
 template <typename T>
 static gbg_status uxw_fused(gbg_plan* p, const cpx<T>* uh, cpx<T>* out, const Geo& g,
-                            double kc, char* base, bool noalloc, hipStream_t s,
+                            const SlabCfg& sc, double kc, char* base, bool noalloc, hipStream_t s,
                             const Epi<T>* ep = nullptr) {
   const size_t cb = sizeof(cpx<T>);
   // W1: K1's 6 compact-ky fields, later K3's 3 full-y fields;
   // W2: K2's 6 full-y fields, later K4's 3 compact-ky fields.
   // With the z-slab chain: W1 the compact fields (K4 writes its 3 into the
   // first three), W2 the ring of slots.
-  const SlabCfg sc = slab_cfg(p, g);
   size_t b1, b2;
   uxw_buffers(g, sc, cb, &b1, &b2);
   gbg_status st = GBG_OK;
@@ -1646,7 +1605,7 @@ size_t uxw_work_bytes(gbg_plan* p, double kc, unsigned flags) {
   // W1, W2 and the K1 partials (2 doubles per launched workgroup, at most
   // one per tile plus the XCD padding)
   size_t b1, b2;
-  uxw_buffers(g, slab_cfg(p, g), cs, &b1, &b2);
+  uxw_buffers(g, slab_cfg(p, g, flags), cs, &b1, &b2);
   return b1 + b2 + sizeof(double) * 2 * ((size_t)g.nyc * g.nhp + fast::kXcd);
 }
 
@@ -1718,14 +1677,15 @@ static gbg_status uxw_run(gbg_plan* p, const void* u_hat, void* n_hat, double kc
                           char* work, bool noalloc, hipStream_t s, const UxwComb* cb = nullptr) {
   if (!uxw_fusable(p, flags)) return uxw_composed(p, u_hat, n_hat, kc, work, s, cb);
   const Geo g = make_geo(p, kc);
+  const SlabCfg sc = slab_cfg(p, g, flags);
   if (p->elem == 8) {
     const Epi<double> e = cb ? make_epi<double>(*cb, n_hat != nullptr) : Epi<double>{};
-    return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, kc, work,
+    return uxw_fused<double>(p, (const cpx<double>*)u_hat, (cpx<double>*)n_hat, g, sc, kc, work,
                              noalloc, s, cb ? &e : nullptr);
   }
   const Epi<float> e = cb ? make_epi<float>(*cb, n_hat != nullptr) : Epi<float>{};
-  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, kc, work, noalloc,
-                          s, cb ? &e : nullptr);
+  return uxw_fused<float>(p, (const cpx<float>*)u_hat, (cpx<float>*)n_hat, g, sc, kc, work,
+                          noalloc, s, cb ? &e : nullptr);
 }
 
 gbg_status pipe_uxw_comb(gbg_plan* p, const void* u_hat, void* n_hat, double kc, const UxwComb& cb,

@@ -201,8 +201,14 @@ gbg_status gbg_reality_projection(gbg_plan* plan, void* dst, const void* src, in
  * energy / enstrophy (host pointers, may be NULL) receive 1/2<u,u> and
  * 1/2<w,w> of the truncated fields in FP64; when either is non-NULL the call
  * synchronises the stream.  flags: GBG_UXW_UNFUSED runs the composition of
- * the standalone operators instead of the fused pass schedule. */
+ * the standalone operators instead of the fused pass schedule;
+ * GBG_UXW_SLABS runs the fused schedule's three middle passes z-slab by
+ * z-slab over two small slots (on two streams, forked from and joined back
+ * into `stream`), which drops the work area's six full-y fields: the same
+ * results bit for bit, 4.4 GB less work area at 512^3 FP64 and 35 GB less at
+ * 1024^3 FP64, and within 1 % of the default time on the MI355X. */
 #define GBG_UXW_UNFUSED 1u
+#define GBG_UXW_SLABS 2u
 gbg_status gbg_uxw(gbg_plan* plan, const void* u_hat, void* n_hat, double kc, double* energy,
                    double* enstrophy, unsigned flags, void* stream);
 /* gbg_uxw for callers that cannot allocate device memory (code under HIP

@@ -80,3 +80,18 @@ def test_python_mirror_refusals_without_gpu(gbg):
     with pytest.raises(gbg.ArgumentError):
         gbg.spectral_band(None, float("nan"), 1.0)
     assert np.isclose(gbg.dealias_cutoff(64), 64 / 3)
+
+
+def test_python_constants_match_the_header():
+    """Every #define GBG_... integer of include/gbg.h that the Python binding
+    mirrors carries the header's value."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "gbg_lib_consts", os.path.join(REPO, "gbgperiodicfields.jl_amd", "_lib.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    defs = dict(re.findall(r"#define\s+(GBG_[A-Z0-9_]+)\s+(\d+)u?\b", open(HEADER).read()))
+    mirrored = [k for k in defs if hasattr(mod, k)]
+    assert {"GBG_UXW_UNFUSED", "GBG_UXW_SLABS"} <= set(mirrored)
+    for k in mirrored:
+        assert getattr(mod, k) == int(defs[k]), k
