@@ -170,13 +170,29 @@ __device__ __forceinline__ void twiddle_ts(cpx<T>* a, int k, int ls, const cpx<T
 // banks of 4 bytes; 16-byte accesses are served 8 lanes per cycle, 8-byte
 // accesses 16).  Offsets that are multiples of 8 map linearly, so every
 // access is a per-thread base plus a compile-time offset.
+//
+// PD = 2 leaves one empty slot after every 16 (slot e + e/16), for 8-byte
+// entries in the x passes, where consecutive lanes hold consecutive elements:
+// an 8-byte access serves 16 lanes per cycle, and with PD = 1 sixteen
+// consecutive elements span 17 slots, so every unit-stride stage read (and
+// the writes of the stage with LS = 64) puts two lanes on one bank pair.
+// With PD = 2 those are conflict-free and so are the first stage's stride-8
+// writes; the writes of the stage with LS = 8 keep a two-way conflict on a
+// quarter of their lanes.  Offsets that are multiples of 16 map linearly; the
+// LS = 8 stage's offsets 8 r (added to a base below 8 mod 16) map to
+// 8 r + r / 2 (wr_off).
 template <int PD>
 __device__ __forceinline__ int slot(int e) {
-  return PD ? e + (e >> 3) : e;
+  return PD == 2 ? e + (e >> 4) : (PD ? e + (e >> 3) : e);
 }
 template <int PD>
 constexpr int slot_step(int c) {
-  return PD ? c + c / 8 : c;
+  return PD == 2 ? c + c / 16 : (PD ? c + c / 8 : c);
+}
+// slot offset of output r of a stage with LS > 1, from slot((j - k) R + k)
+template <int PD, int LS>
+constexpr int wr_off(int r) {
+  return PD == 2 && LS == 8 ? 8 * r + r / 2 : r * slot_step<PD>(LS);
 }
 
 // Stage (R, LS) of the ping-pong schedule: exchange k (k = 0, 1, ...) writes
@@ -197,6 +213,9 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
   constexpr bool LAST = (LS * R == N);
   static_assert(!PD || ((FIRST || NR % 8 == 0) && (LAST || LS % 8 == 0 || (LS == 1 && R == 8))),
                 "slot offsets must be multiples of 8");
+  static_assert(PD != 2 || ((FIRST || NR % 16 == 0) && (LAST || LS == 8 || LS % 16 == 0 ||
+                                                       (LS == 1 && R == 8))),
+                "PD = 2: read offsets multiples of 16, write offsets as wr_off");
   cpx<T>* const rd = W == 0 ? p1 : p0;   // written by the previous stage
   cpx<T>* const pen = W == 0 ? p0 : p1;  // written by this stage
   cpx<T> a[NB][R];
@@ -237,10 +256,10 @@ __device__ __forceinline__ void stages(cpx<T> (&v)[E], cpx<T>* p0, cpx<T>* p1, i
     for (int b = 0; b < NB; ++b) {
       const int j = t + TP * b;
       const int k = j % LS;
-      // LS = 1: slot(8 j + r) = 9 j + r; LS >= 8: slot(o + r LS) = slot(o) + r slot_step(LS)
+      // LS = 1: slot(8 j + r) = slot(8 j) + r; LS >= 8: slot(o + r LS) = slot(o) + wr_off(r)
       cpx<T>* o = pen + slot<PD>((j - k) * R + k);
 #pragma unroll
-      for (int r = 0; r < R; ++r) o[r * (LS == 1 ? 1 : slot_step<PD>(LS))] = a[b][r];
+      for (int r = 0; r < R; ++r) o[LS == 1 ? r : wr_off<PD, LS>(r)] = a[b][r];
     }
     lds_sync<SYNC>();
     stages<T, N, E, LS * R, DIR, SYNC, 1 - W, PD, ONE, TS>(v, p0, p1, t, tw);
@@ -298,9 +317,10 @@ struct Cfg {
   static constexpr int U = XTP >= 256 ? 1 : 256 / XTP;
   static constexpr int XNT = U * XTP;
   static constexpr int XSYNC = XTP <= 64 ? 1 : 0;
-  // x pass: consecutive lanes hold consecutive elements of one pencil
-  static constexpr int XPD = 1;
-  static constexpr int XLD = N + N / 8;
+  // x pass: consecutive lanes hold consecutive elements of one pencil (slot
+  // padding per 16 entries for FP32 where every stage read spans 16 entries)
+  static constexpr int XPD = (sizeof(T) == 4 && N >= 128) ? 2 : 1;
+  static constexpr int XLD = XPD == 2 ? N + N / 16 : N + N / 8;
 };
 
 
