@@ -107,7 +107,7 @@ static gbg_status ops_on(const int n[3], int elem, const int dims[3]) {
   TRY(gbg_reality_projection(pl.h, spec3b, spec3b, 3, nullptr));
   double E = 0, Z = 0;
   const double kc = 2.0;
-  for (unsigned flags : {0u, (unsigned)GBG_UXW_UNFUSED}) {
+  for (unsigned flags : {0u, (unsigned)GBG_UXW_UNFUSED, (unsigned)GBG_UXW_SLABS}) {
     TRY(gbg_uxw(pl.h, spec3, out3, kc, &E, &Z, flags, nullptr));
     gbg_uxw_combination cb;
     cb.dst = spec3b;
@@ -143,6 +143,42 @@ static gbg_status sc_ops_fused32() {
   const int n[3] = {64, 32, 32};
   return ops_on(n, 4, nullptr);
 }
+// the z-slab chain on several slabs and two streams: GBG_UXW_SLAB at plan
+// creation sets 5 planes per slab (a last slab of 4) and two slots
+static gbg_status sc_uxw_slabs(int elem) {
+  const int n[3] = {32, 32, 64};
+  const double L[3] = {6.283185307179586, 6.283185307179586, 6.283185307179586};
+  Plan pl;
+  setenv("GBG_UXW_SLAB", "5,2", 1);
+  const gbg_status cs = gbg_plan_create(n, L, elem, &pl.h);
+  unsetenv("GBG_UXW_SLAB");
+  TRY(cs);
+  size_t M = 0, P = 0, dev = 0;
+  TRY(gbg_plan_query(pl.h, &M, &P, &dev));
+  const size_t sc = M * 2 * elem;
+  Bufs b;
+  void *spec3, *out3, *dst3;
+  TRY(b.get(&spec3, 3 * sc));
+  TRY(b.get(&out3, 3 * sc));
+  TRY(b.get(&dst3, 3 * sc));
+  TRY(gbg_memset(spec3, 0, 3 * sc, nullptr));
+  double E = 0, Z = 0;
+  TRY(gbg_uxw(pl.h, spec3, out3, 2.0, &E, &Z, GBG_UXW_SLABS, nullptr));
+  TRY(gbg_uxw(pl.h, spec3, out3, 2.0, nullptr, nullptr, GBG_UXW_SLABS, nullptr));
+  gbg_uxw_combination cb;
+  cb.dst = dst3;
+  cb.a = spec3;
+  cb.alpha = 1.0;
+  cb.b = nullptr;
+  cb.beta = 0.0;
+  cb.delta = 0.01;
+  cb.gamma = 0.0;
+  cb.coefficient = -0.001;
+  TRY(gbg_uxw_combine(pl.h, spec3, out3, 2.0, &cb, &E, &Z, GBG_UXW_SLABS, nullptr));
+  return GBG_OK;
+}
+static gbg_status sc_uxw_slabs64() { return sc_uxw_slabs(8); }
+static gbg_status sc_uxw_slabs32() { return sc_uxw_slabs(4); }
 static gbg_status sc_ops_composed() {
   const int n[3] = {16, 12, 10};
   return ops_on(n, 8, nullptr);
@@ -291,6 +327,8 @@ int main() {
     std::function<gbg_status()> run;
   } scs[] = {{"ops fused FP64 32^3", sc_ops_fused64},
              {"ops fused FP32 64x32x32", sc_ops_fused32},
+             {"uxw slabs FP64 32x32x64", sc_uxw_slabs64},
+             {"uxw slabs FP32 32x32x64", sc_uxw_slabs32},
              {"ops composed 16x12x10", sc_ops_composed},
              {"ops layout (3,2,1) 8x7x6", sc_ops_layout},
              {"ops chirp-z FP32 17x19x6", sc_ops_chirp},
