@@ -273,7 +273,7 @@ static gbg_status create_plan(const int n[3], const double L[3], int elem, const
     p->generic_only = (env && env[0] == '1') ? 1 : 0;
     const char* wide = std::getenv("GBG_WIDE_OFFSETS");
     p->wide_offsets = (wide && wide[0] == '1') ? 1 : 0;
-    p->x_var = n[0] >= 1024 ? 1 : 0;   // the x pass's default shape (gbg_uxw.hip)
+    p->x_var = x_default_shape(n[0], elem);
     const char* xs = std::getenv("GBG_X_SHAPE");
     if (xs && (xs[0] == '0' || xs[0] == '1') && xs[1] == 0) p->x_var = xs[0] - '0';
     // the z-slab chain of the fused pipelines (gbg_uxw.hip): Z,R or 0 (off)

@@ -128,7 +128,7 @@ struct gbg_plan {
   int use_fast[3];
   int bsel[3];
   // tile shape of the fused pipelines' x pass (0: the default row-pair units
-  // per workgroup, 1: half as many); chosen at creation from the x length,
+  // per workgroup, 1: half as many); chosen at creation (x_default_shape),
   // changed by gbg_plan_autotune or GBG_X_SHAPE=0|1 at creation
   int x_var;
   // FP64 per-axis tables (device): derivative family and squared family
@@ -168,6 +168,10 @@ struct gbg_plan {
 };
 
 namespace gbg {
+
+// the x-pass tile shape a plan starts with (gbg_uxw.hip, XCfg): half-size
+// row-pair groups only for FP64 rows of 1024 and longer, which span two waves
+inline int x_default_shape(int nx, int elem) { return (nx >= 1024 && elem == 8) ? 1 : 0; }
 
 // status plumbing --------------------------------------------------------------
 void set_error(const std::string& msg);

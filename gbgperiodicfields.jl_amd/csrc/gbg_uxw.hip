@@ -58,13 +58,14 @@ struct Geo {
 using fast::Cfg;
 
 // Tile shapes of the x pass (K3, the product's P3), chosen per plan
-// (gbg_plan::x_var): V = 0 the row-pair units of Cfg per workgroup; V = 1 half
-// as many, so more, smaller workgroups share a CU.  Measured per pass on the
-// MI355X (u x omega, 2/3 cutoff): at Nx = 1024 V = 1 is 5 % faster in FP64
-// (18.1 -> 17.1 ms) and FP32 (9.37 -> 8.87 ms); at Nx = 512 it is 9 % slower
-// in FP64 and within run-to-run spread in FP32.  (A larger staging area, and
-// for the z-inverse pass half the pencils per workgroup or sixteen elements
-// per thread, measured slower or spilled, and are not built.)
+// (gbg_plan::x_var, default x_default_shape): V = 0 the row-pair units of Cfg
+// per workgroup; V = 1 half as many, so more, smaller workgroups share a CU.
+// Measured per pass on the MI355X (u x omega, 2/3 cutoff): at Nx = 1024 in
+// FP64 (two waves per row, no staging) V = 1 is 5 % faster (18.1 -> 17.1 ms);
+// in FP32 at Nx = 1024, staged, V = 0 is 6 % faster (8.85 -> 8.34 ms); at
+// Nx = 512 V = 1 is 9 % slower in FP64 and within run-to-run spread in FP32.
+// (For the z-inverse pass half the pencils per workgroup or sixteen elements
+// per thread measured slower or spilled, and are not built.)
 template <typename T, int N, int V>
 struct XCfg {
   using D = Cfg<T, N>;
@@ -72,8 +73,6 @@ struct XCfg {
   static constexpr int U = V == 1 ? (D::U > 1 ? D::U / 2 : 1) : D::U;
   static constexpr int XNT = U * XTP;
 };
-// the x-pass shape a plan starts with (gbg_plan_autotune may change it)
-constexpr int x_default_shape(int n) { return n >= 1024 ? 1 : 0; }
 
 // With OPAQUE, the same pointer made opaque to the optimiser: table loads
 // behind it stay where they are written instead of being hoisted (and held in
@@ -568,7 +567,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 template <typename T, int N, int V>
 struct XStage {
   static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
-  static constexpr int CH = sizeof(T) == 8 ? 3 : 2;            // copies per field row
+  // copies per field row: three in FP64 and in FP32 at N >= 1024, so the kept
+  // kx range of the 2/3 cutoff (N/3 + 1 entries) fits the staging area
+  static constexpr int CH = (sizeof(T) == 8 || N >= 1024) ? 3 : 2;
   static constexpr int TPE = Cfg<T, N>::XTP * EPL;             // entries per copy per unit
   static constexpr int CAP = CH * TPE;                         // staged entries per row
   static constexpr int WAVE_ELEMS = 2 * CH * 64 * EPL;         // staging elements per wave

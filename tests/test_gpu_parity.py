@@ -249,6 +249,9 @@ UXW_CASES = [((16, 16, 16), None), ((32, 32, 32), None), ((64, 64, 64), None), (
              ((64, 64, 128), None), ((128, 128, 32), None), ((64, 64, 64), 5.0), ((64, 64, 64), 1000.0),
              ((32, 32, 64), 7.3), ((512, 32, 32), 170.5), ((256, 64, 32), 60.0),
              ((1024, 32, 32), 340.0), ((32, 1024, 32), 12.0), ((32, 32, 1024), 12.0),
+             # Nx = 1024: 384 kept kx fill the FP32 x pass's staging area exactly; 501
+             # exceed it (the unstaged form)
+             ((1024, 32, 32), 383.5), ((1024, 32, 32), 500.0),
              # Ny = 512: the y passes of the 512^3 headline, k_uxw_y<T, 512, +-1, ...>
              ((64, 512, 64), None), ((32, 512, 64), 150.0)]
 
