@@ -298,7 +298,11 @@ struct Cfg {
   // strided passes: E elements per thread, TP threads per pencil
   static constexpr int E = 8;
   static constexpr int TP = N / E;
-  static constexpr int WG = sizeof(T) == 8 ? 512 : 1024;
+  // workgroup threads: at most 64 pencils, so at short lengths a smaller
+  // workgroup means more, smaller tiles (still >= 128-byte segments); the
+  // persistent FP64 z passes at N >= 128 expect one 512-thread workgroup per CU
+  static constexpr int WG = sizeof(T) == 8 ? (N <= 64 ? 256 : 512)
+                                           : (N <= 128 ? 256 : (N <= 256 ? 512 : 1024));
   static constexpr int B0 = WG / TP;
   static constexpr int B = B0 < 1 ? 1 : (B0 > 64 ? 64 : B0);
   static constexpr int NT = B * TP;
