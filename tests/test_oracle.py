@@ -162,6 +162,29 @@ def test_helmholtz_projector_properties():
     assert np.max(np.abs(O.helmholtz_projection(O.gradient(s, m), m))) < 1e-11
 
 
+def test_helmholtz_kernel_and_weights_compose_from_pinned_parts():
+    """The kernel and alpha/beta forms of helmholtz_projection!
+    (src/DifferentialOperators.jl:191-230: dst = alpha K P(u) + beta dst) equal
+    the band convolve! (pinned by the lattice counts and mask digests) applied
+    to the plain projector (pinned above by idempotence, zero divergence and
+    annihilated gradients), on an anisotropic box; the result stays solenoidal
+    and vanishes outside the band."""
+    g = O.Grid3D((12, 10, 8), (O.TWO_PI, 3.0, 5.0))
+    m = O.Multipliers.build(g)
+    rng = np.random.default_rng(8)
+    u = O.to_spectral(rng.standard_normal((3,) + g.physical_shape))
+    d = O.to_spectral(rng.standard_normal((3,) + g.physical_shape))
+    lo, hi, a, b = 1.5, 7.0, 0.75, -0.4
+    K = O.band_mask(m, lo, hi)
+    got = O.helmholtz_projection(u, m, kernel=K, alpha=a, beta=b, dst=d)
+    want = O.convolve_band(O.helmholtz_projection(u, m), m, lo, hi, alpha=a, beta=b, dst=d)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13 * np.max(np.abs(u)))
+    plain = O.helmholtz_projection(u, m, kernel=K, alpha=a)
+    assert np.max(np.abs(O.divergence(plain, m))) < 1e-12 * np.max(np.abs(u))
+    assert np.all(plain[:, K == 0] == 0)
+    assert 0 < K.sum() < K.size
+
+
 def test_inverse_curl_roundtrip():
     g = O.Grid3D((8, 8, 6))
     m = O.Multipliers.build(g)
