@@ -357,8 +357,8 @@ __device__ __forceinline__ void c2r_pair_fetch(cpx<T> (&a)[E], cpx<T> (&b)[E],
     a[m] = mk<T>(T(0), T(0));
     b[m] = mk<T>(T(0), T(0));
     if (kk < g.nhk) {
-      if (oka) a[m] = ra[kk];
-      if (okb) b[m] = rb[kk];
+      if (oka) a[m] = ld_s(ra + kk);
+      if (okb) b[m] = ld_s(rb + kk);
     }
   }
 }
@@ -406,8 +406,8 @@ __device__ __forceinline__ void r2c_pair_store(cpx<T> (&v)[E], cpx<T>* pen,
   for (int k = t; k < g.nhk; k += C::XTP) {
     const cpx<T> z = pen[k];
     const cpx<T> zc = pen[k == 0 ? 0 : N - k];
-    if (oka) ra[k] = mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y));
-    if (okb) rb[k] = mk<T>(T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x));
+    if (oka) st_s(ra + k, mk<T>(T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)));
+    if (okb) st_s(rb + k, mk<T>(T(0.5) * (z.y + zc.y), T(0.5) * (zc.x - z.x)));
   }
   lds_sync<C::XSYNC>();
 }

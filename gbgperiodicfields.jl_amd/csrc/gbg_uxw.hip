@@ -207,7 +207,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       for (int m = 0; m < E; ++m) {
         Off zi = (Off)off + (keep[m] ? (Off)(t + TP * m) * (Off)plane : (Off)0);
         asm volatile("" : "+v"(zi));
-        r[m] = u[zi];
+        r[m] = ld_s(u + zi);
       }
     };
     auto emit = [&](cpx<T> (&v)[E], int field) {
@@ -218,7 +218,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
         for (int m = 0; m < E; ++m) {
           Off zo = (Off)p1 + (Off)(t + TP * m) * (Off)zs;
           asm volatile("" : "+v"(zo));
-          o[zo] = v[m];
+          st_s(o + zo, v[m]);
         }
       }
       asm volatile("" ::: "memory");
@@ -449,6 +449,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
   //    twiddles from global memory (at N = 1024 the written-out form and the
   //    LDS or opaque twiddle reads each cost about a quarter more time).
   constexpr bool STREAM = DIR > 0 && N <= 512;
+  // nontemporal stores up to N = 512; at 1024 they made both y passes slower
+  // (1024^3 FP64 K2 10.8 -> 11.5 ms, K4 5.95 -> 6.23 ms)
+  constexpr bool NTS = (GBG_NT & 2) != 0 && N <= 512;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   const long long tile = fast::xcd_tile();
@@ -507,9 +510,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
       if constexpr (STREAM) {
         long long zi = held(m) ? idx(m) : 0;
         asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
-        v[m] = in[zi];
+        v[m] = ld_s(in + zi);
       } else {
-        v[m] = held(m) ? in[idx(m)] : mk<T>(T(0), T(0));
+        v[m] = held(m) ? ld_s(in + idx(m)) : mk<T>(T(0), T(0));
       }
     }
   };
@@ -523,13 +526,13 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
       if (st_ok) {
         cpx<T>* o = dst + f * g.s2 + fbase;
 #pragma unroll
-        for (int m = 0; m < E; ++m) o[(long long)(t + TP * m) * g.nhp] = v[m];
+        for (int m = 0; m < E; ++m) st_s<NTS>(o + (long long)(t + TP * m) * g.nhp, v[m]);
       }
     } else {
       cpx<T>* o = dst + f * g.s1 + cbase;
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        if (coff[m] >= 0) o[coff[m]] = v[m];
+        if (coff[m] >= 0) st_s<NTS>(o + coff[m], v[m]);
     }
   };
   load(0, nxt);
@@ -564,6 +567,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
 // PROD (the dealiased scalar product): the two fields a, b of a row share one
 // complex transform, p = a b is formed in registers, and p of the two rows of
 // the unit share one forward transform (one output field).
+// cache-policy bits of the staging copies (2: nontemporal)
+constexpr int kLdsAux = (GBG_NT & 4) != 0 ? 2 : 0;
+
 template <typename T, int N, int V>
 struct XStage {
   static constexpr int EPL = 16 / (int)sizeof(cpx<T>);        // entries per lane per copy
@@ -619,10 +625,10 @@ __global__ __launch_bounds__((XCfg<T, N, V>::XNT), 2) void k_uxw_x(const cpx<T>*
         const int e = e0 < g.nhk ? e0 : 0;   // entries past nhk are never read back
         __builtin_amdgcn_global_load_lds(
             (const void __attribute__((address_space(1)))*)(w2 + q * g.s2 + ro + e),
-            (void __attribute__((address_space(3)))*)(sw + c * 64 * X::EPL), 16, 0, 0);
+            (void __attribute__((address_space(3)))*)(sw + c * 64 * X::EPL), 16, 0, kLdsAux);
         __builtin_amdgcn_global_load_lds(
             (const void __attribute__((address_space(1)))*)(w2 + (q + QOFF) * g.s2 + ro + e),
-            (void __attribute__((address_space(3)))*)(sw + (X::CH + c) * 64 * X::EPL), 16, 0, 0);
+            (void __attribute__((address_space(3)))*)(sw + (X::CH + c) * 64 * X::EPL), 16, 0, kLdsAux);
       }
     }
   };
@@ -861,7 +867,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
               if (ep.store_n) nh[co] = mk<T>(T(0), T(0));
               epi_store(ep, co, qxy + tb.ks3[t + TP * m], mk<T>(T(0), T(0)));
             } else {
-              nh[co] = mk<T>(T(0), T(0));
+              st_s(nh + co, mk<T>(T(0), T(0)));
             }
           }
       }
@@ -889,9 +895,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
         if constexpr (STREAM) {
           long long zi = disk ? (long long)(t + TP * m) * zs : 0;
           asm("" : "+v"(zi));  // keeps the optimiser from splitting the load by cases
-          n[c][m] = in[zi];
+          n[c][m] = ld_s(in + zi);
         } else {
-          n[c][m] = disk ? in[(long long)(t + TP * m) * zs] : mk<T>(T(0), T(0));
+          n[c][m] = disk ? ld_s(in + (long long)(t + TP * m) * zs) : mk<T>(T(0), T(0));
         }
       }
     }
@@ -909,7 +915,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const long long o = (long long)(t + TP * m) * plane + p5;
-        nh[o] = mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0));
+        st_s(nh + o, mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0)));
       }
     } else {
       const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
@@ -942,9 +948,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
           epi_store(ep, g.M + o, q, r2);
           epi_store(ep, 2 * g.M + o, q, r3);
         } else {
-          nh[o] = r1;
-          nh[g.M + o] = r2;
-          nh[2 * g.M + o] = r3;
+          st_s(nh + o, r1);
+          st_s(nh + g.M + o, r2);
+          st_s(nh + 2 * g.M + o, r3);
         }
       }
     }
@@ -987,7 +993,7 @@ __global__ __launch_bounds__(256) void k_uxw_zero_tiles(cpx<T>* __restrict__ nh,
     cpx<T>* out = nh + (long long)pl * plane;
 #pragma unroll
     for (int i = 0; i < R; ++i)
-      if (z[i]) out[base + i * 256] = zero;
+      if (z[i]) st_s(out + base + i * 256, zero);
   }
 }
 
@@ -1241,8 +1247,7 @@ static const TileSet* tile_set(gbg_plan* p, const Geo& g, int b, double kc, hipS
 // pencil) depend on nothing the other passes compute, and the transform
 // kernel writes only the other tiles.  When the output overlaps no input
 // they run on the plan's second stream beside one of the first passes
-// (zero_fork_pass: K1 in FP32, K3 in FP64 with the integrating-factor
-// combination, none in FP64 otherwise), forked from s before that pass
+// (zero_fork_pass: K1 by default), forked from s before that pass
 // and joined back into s after the last (zero_tiles_join); whenever the fork
 // is not taken the last pass launches them on s itself.  Nor is the fork
 // taken while s is being captured into a graph: the batched form's graph of
@@ -1295,21 +1300,21 @@ static void launch_zero_tiles(const Geo& g, const TileSet* ts, int b, int nc, cp
 }
 
 // The pass of u x omega beside which the zero tiles run on the second stream
-// (0: none, in stream order in the last pass).  Measured per term (min of
-// three runs of 20): 512^3 FP64 none 5.333, beside K1 5.327, K2 5.372, K3
-// 5.309, K4 5.358 ms; 1024^3 FP64 none 53.90, K1 54.17, K3 53.72 ms; 512^3
-// FP32 none 2.880, K1 2.840, K3 2.879 ms; 1024^3 FP32 all three within
-// 0.05 ms.  In FP64 the gain beside K3 is under 0.5 % for the plain term,
-// where the zero tiles only write (and it would fold their time into K3's
-// measured span), but 2.5 % for the integrating-factor step, whose zero-tile
-// kernel reads two fields and forms the exponential factor: FP64 forks them
-// beside K3 with the combination only.  GBG_ZERO_FORK_F64 / _F32 = 0..4
-// override the choice (the results are the same bits).
+// (0: none, in stream order in the last pass).  With nontemporal stores in the
+// passes, measured per term on the MI355X: 512^3 FP64 none 5.15, beside K1
+// 5.09, beside K3 5.14 ms; 1024^3 FP64 none 53.44, K1 53.43, K3 53.03 ms;
+// 512^3 FP32 none 2.77, K1 2.73 ms; the integrating-factor RK2 step at 512^3
+// FP64 beside K3 13.78, beside K1 13.55 ms.  K1 (one persistent workgroup per
+// CU, bound by latency) leaves the memory system room; its time then includes
+// the zero tiles' writes, and the last pass is the transform kernel alone.
+// GBG_ZERO_FORK_F64 / _F32 = 0..4 override the choice (the results are the
+// same bits).
 template <typename T>
 static int zero_fork_pass(bool combination) {
+  (void)combination;
   const char* e = std::getenv(sizeof(T) == 8 ? "GBG_ZERO_FORK_F64" : "GBG_ZERO_FORK_F32");
   if (e && e[0] >= '0' && e[0] <= '4' && e[1] == 0) return e[0] - '0';
-  return sizeof(T) == 8 ? (combination ? 3 : 0) : 1;
+  return 1;
 }
 
 // fork point: recorded on s before the pass the zero tiles run beside
