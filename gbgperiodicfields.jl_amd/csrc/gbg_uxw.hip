@@ -142,9 +142,17 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
                        160 * 1024;
   __shared__ cpx<T> twl[TWL ? N : 1];
   __shared__ T a3l[N];
+  // the kz squared family as well where LDS holds it, in FP64 (the keep bits
+  // of a tile then need no global read; in FP32 the LDS reads spilled at the
+  // 128-VGPR limit)
+  constexpr bool Q3L = sizeof(T) == 8 && (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + 2 * (C::NT / 64) * sizeof(double) +
+                           (TWL ? N * sizeof(cpx<T>) : 0) + (size_t)N * (sizeof(T) + sizeof(double)) <=
+                       160 * 1024;
+  __shared__ double q3l[Q3L ? N : 1];
   for (int i = threadIdx.x; i < N; i += C::NT) {
     if constexpr (TWL) twl[i] = tw[i];
     a3l[i] = T(tb.kd3[i]);
+    if constexpr (Q3L) q3l[i] = tb.ks3[i];
   }
   auto twp = [&]() {
     if constexpr (TWL)
@@ -152,14 +160,58 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     else
       return tw;
   };
+  auto q3 = [&](int kz) {
+    if constexpr (Q3L)
+      return q3l[kz];
+    else
+      return tb.ks3[kz];
+  };
   double eacc = 0.0, zacc = 0.0;
+  // FP64: the next tile's table values are read while a tile runs (below);
+  // in FP32 the values held across the tile spill at the 128-VGPR limit
+  constexpr bool PREF = sizeof(T) == 8 && z_inverse_persist<T, N, O32>();
+  // a tile's per-pencil table values (read from global memory; with PREF
+  // only, else run_tile reads them itself)
+  struct TileGeo {
+    int kx, ky;
+    double qxy;
+    T a1, a2;
+    bool disk;
+  };
+  auto geo_of = [&](long long tile) {
+    const long long p1 = tile * B + pl;
+    TileGeo q;
+    q.kx = 0;
+    q.ky = 0;
+    q.qxy = 0.0;
+    q.disk = false;
+    if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
+      const int kyc = (int)(p1 / g.nhp);
+      q.kx = (int)(p1 - (long long)kyc * g.nhp);
+      if (q.kx < g.nhk) {
+        q.ky = ky_of(kyc, g);
+        q.qxy = tb.ks1[q.kx] + tb.ks2[q.ky];
+        q.disk = q.qxy <= g.hi2;
+      } else {
+        q.kx = 0;
+      }
+    }
+    q.a1 = T(tb.kd1[q.kx]);
+    q.a2 = T(tb.kd2[q.ky]);
+    return q;
+  };
   // one tile: B consecutive kx of one compact (kyc) row
-  auto run_tile = [&](long long tile) {
+  auto run_tile = [&](long long tile, const TileGeo& q) {
     const long long p1 = tile * B + pl;
     int kx = 0, ky = 0;
     double qxy = 0.0;
     bool disk = false;
-    if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
+    if constexpr (PREF) {
+      kx = q.kx;
+      ky = q.ky;
+      qxy = q.qxy;
+      disk = q.disk;
+    } else if (tile < ntiles && p1 < (long long)g.nyc * g.nhp) {
       const int kyc = (int)(p1 / g.nhp);
       kx = (int)(p1 - (long long)kyc * g.nhp);
       if (kx < g.nhk) {
@@ -172,13 +224,13 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     }
     if (!__syncthreads_or(disk)) return;
     double ea = 0.0, za = 0.0;   // this tile's sums (local: no pointer to the outer ones)
-    const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
+    const T a1 = PREF ? q.a1 : T(tb.kd1[kx]), a2 = PREF ? q.a2 : T(tb.kd2[ky]);
     const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
     const long long plane = (long long)g.ny * g.nh;
     const long long off = (long long)ky * g.nh + kx;
     bool keep[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + tb.ks3[t + TP * m] <= g.hi2);  // no branch
+    for (int m = 0; m < E; ++m) keep[m] = disk & (qxy + q3(t + TP * m) <= g.hi2);  // no branch
     cpx<T>* pen = lds + pl * LD;
     cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
     const long long zs = (long long)g.nyc * g.nhp;
@@ -298,11 +350,24 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const int xcd = blockIdx.x % fast::kXcd;
     const long long L = gridDim.x / fast::kXcd;
     const long long tend = (xcd + 1) * per < ntiles ? (xcd + 1) * per : ntiles;
+    long long tile = xcd * per + blockIdx.x / fast::kXcd;
+    if constexpr (PREF) {
+      // the next tile's table values are read while this tile runs: read at
+      // the next tile's start, their wait would also cover this tile's last
+      // stores (loads and stores retire through one in-order counter)
+      TileGeo cur = geo_of(tile < tend ? tile : ntiles);
 #pragma unroll 1
-    for (long long tile = xcd * per + blockIdx.x / fast::kXcd; tile < tend; tile += L)
-      run_tile(tile);
+      for (; tile < tend; tile += L) {
+        const TileGeo nxt = geo_of(tile + L < tend ? tile + L : ntiles);
+        run_tile(tile, cur);
+        cur = nxt;
+      }
+    } else {
+#pragma unroll 1
+      for (; tile < tend; tile += L) run_tile(tile, TileGeo{});
+    }
   } else {
-    run_tile(fast::xcd_tile());
+    run_tile(fast::xcd_tile(), TileGeo{});
   }
   // FP64 block reduction of the energy / enstrophy partial sums, fixed order
   for (int s = 32; s > 0; s >>= 1) {
@@ -830,8 +895,27 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
   constexpr bool TWL =
       STREAM && (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + (size_t)N * sizeof(cpx<T>) <= 160 * 1024;
   __shared__ cpx<T> twl[TWL ? N : 1];
-  if constexpr (TWL)
-    for (int i = threadIdx.x; i < N; i += C::NT) twl[i] = tw[i];
+  // The kz derivative factors (and, where LDS holds them too, the kz squared
+  // family) are read from LDS copies: a global table read between the stores
+  // of one kz and the next waits for every store issued before it.  Both
+  // copies are complete before their first use, which follows the barrier of
+  // the first tile's disk vote.
+  __shared__ T a3l[N];
+  constexpr bool Q3L = (size_t)C::NBUF * B * LD * sizeof(cpx<T>) + (TWL ? N * sizeof(cpx<T>) : 0) +
+                           (size_t)N * (sizeof(T) + sizeof(double)) <=
+                       160 * 1024;
+  __shared__ double q3l[Q3L ? N : 1];
+  for (int i = threadIdx.x; i < N; i += C::NT) {
+    if constexpr (TWL) twl[i] = tw[i];
+    a3l[i] = T(tb.kd3[i]);
+    if constexpr (Q3L) q3l[i] = tb.ks3[i];
+  }
+  auto q3 = [&](int kz) {
+    if constexpr (Q3L)
+      return q3l[kz];
+    else
+      return tb.ks3[kz];
+  };
   auto twp = [&]() {
     if constexpr (TWL)
       return (const cpx<T>*)twl;
@@ -847,12 +931,15 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const int p5 = valid ? (int)p5l : 0;
     int kx = 0, ky = 0, kyc = -1;
     double qxy = 0.0;
+    T a1 = T(0), a2 = T(0);
     bool disk = false;
     if (valid) {
       ky = p5 / g.nh;
       kx = p5 - ky * g.nh;
       kyc = kyc_of(ky, g);
       qxy = tb.ks1[kx] + tb.ks2[ky];
+      a1 = T(tb.kd1[kx]);
+      a2 = T(tb.kd2[ky]);
       disk = kx < g.nhk && kyc >= 0 && qxy <= g.hi2;
     }
     const long long plane = (long long)g.ny * g.nh;
@@ -865,7 +952,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
             const long long co = c * g.M + (long long)(t + TP * m) * plane + p5;
             if constexpr (EPI) {
               if (ep.store_n) nh[co] = mk<T>(T(0), T(0));
-              epi_store(ep, co, qxy + tb.ks3[t + TP * m], mk<T>(T(0), T(0)));
+              epi_store(ep, co, qxy + q3(t + TP * m), mk<T>(T(0), T(0)));
             } else {
               st_s(nh + co, mk<T>(T(0), T(0)));
             }
@@ -881,7 +968,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int kz = t + TP * m;
-      keep[m] = disk & (qxy + tb.ks3[kz] <= g.hi2) &  // no branch
+      keep[m] = disk & (qxy + q3(kz) <= g.hi2) &  // no branch
                 (NC == 1 || !(kx == 0 && ky == 0 && kz == 0));
     }
     // STREAM: components off the disk read the compact buffer's first entry (a
@@ -918,11 +1005,10 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
         st_s(nh + o, mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0)));
       }
     } else {
-      const T a1 = T(tb.kd1[kx]), a2 = T(tb.kd2[ky]);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int kz = t + TP * m;
-        const T a3 = T(tb.kd3[kz]);
+        const T a3 = a3l[kz];
         // (_helmholtz_projection_kernel!, src/DifferentialOperators.jl:191-230), alpha = 1
         const T sq = a1 * a1 + a2 * a2 + a3 * a3;
         const T den = sq == T(0) ? T(1) : sq;
@@ -943,7 +1029,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
             nh[g.M + o] = r2;
             nh[2 * g.M + o] = r3;
           }
-          const double q = qxy + tb.ks3[kz];
+          const double q = qxy + q3(kz);
           epi_store(ep, o, q, r1);
           epi_store(ep, g.M + o, q, r2);
           epi_store(ep, 2 * g.M + o, q, r3);
