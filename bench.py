@@ -94,8 +94,9 @@ def uxw_stage_bytes(n, kc, elem):
     pass must read and write given the index ranges the 2/3 mask keeps
     (complex elements of 2 elem bytes).  With row = Nz Ny nhk (full-y rows of
     kept kx) and disk = Nz x (kept (kx, ky) pencils):
-      K1  read u^ on the kept ball (3 comps), write u, w on the disk (6)
-      K2  read 6 on the disk, write 6 full-y rows
+      K1  read u^ on the kept ball (3 comps), write u and the two kz terms
+          of w on the disk (5; at Ny > 512 u and w, 6)
+      K2  read those on the disk, write 6 full-y rows (u, w)
       K3  read 6 rows, write 3 rows (u x w transformed back)
       K4  read 3 rows, write 3 on the disk
       K5  read 3 on the disk, write the whole output (3 M, zeros included)."""
@@ -104,8 +105,9 @@ def uxw_stage_bytes(n, kc, elem):
     M = (n[0] // 2 + 1) * n[1] * n[2]
     row = n[2] * n[1] * r["nhk"]
     disk = n[2] * r["disk"]
-    return {"K1_z_inverse_curl": (3 * r["ball"] + 6 * disk) * sc,
-            "K2_y_inverse": (6 * disk + 6 * row) * sc,
+    nz1 = 5 if n[1] <= 512 else 6    # fields from the z-inverse to the y-inverse
+    return {"K1_z_inverse_curl": (3 * r["ball"] + nz1 * disk) * sc,
+            "K2_y_inverse": (nz1 * disk + 6 * row) * sc,
             "K3_x_product": 9 * row * sc,
             "K4_y_forward": (3 * row + 3 * disk) * sc,
             "K5_z_forward_project": (3 * disk + 3 * M) * sc}

@@ -10,15 +10,17 @@
 // removes.  With kc the cutoff, only kx <= kx_max, |ky| <= ky_max and the
 // (kx, ky) disk kx^2 + ky^2 <= kc^2 can carry data before the z transforms,
 // so the intermediates are stored compactly over those index ranges:
-//   K1  z-inverse:   u^ (ball of M) -> 6 fields (u, w = curl u^) on the disk
-//                    pencils; energy and enstrophy partial sums in FP64
-//   K2  y-inverse:   6 fields, (z, kx <= kx_max) pencils, loads on the disk only
+//   K1  z-inverse:   u^ (ball of M) -> 5 fields (u and the kz terms of
+//                    w = curl u^) on the disk pencils; energy and enstrophy
+//                    partial sums in FP64
+//   K2  y-inverse:   5 fields -> 6 (u, w: the kx and ky terms of w formed in
+//                    registers), (z, kx <= kx_max) pencils, loads on the disk only
 //   K3  x-pass:      6 C2R rows -> u x w in registers -> 3 R2C rows (kx <= kx_max)
 //   K4  y-forward:   3 fields, stores the disk ky range only
 //   K5  z-forward:   3 fields -> Leray projection, mask, mean-free -> N^ (full
 //                    layout, zeros outside the mask), on the output tiles that
 //                    hold disk pencils; a write-only kernel zeros the other tiles
-// Buffers: W1 = [6][Nz][nyc][nhp] (K1 -> K2), reused as [3][Nz][nyc][nhp]
+// Buffers: W1 = [6][Nz][nyc][nhp] (K1 -> K2, five used), reused as [3][Nz][nyc][nhp]
 // (K4 -> K5); W2 = [6][Nz][Ny][nhp] (K2 -> K3), reused as [3][Nz][Ny][nhp]
 // (K3 -> K4).  nhk = kx_max + 1 entries of each row are used, nhp = nhk
 // rounded up to 16; nyc = number of kept ky indices.
@@ -118,9 +120,20 @@ constexpr bool z_inverse_persist() {
   return sizeof(T) == 8 ? (N >= 128 && N <= 512) : (O32 && N == 512);
 }
 
+// Where the y-inverse pass forms the kx and ky terms of w = curl u^ (the
+// z-inverse then writes five fields: u and the two kz terms of w): Ny <= 512,
+// where the y passes stream.  At Ny = 1024 the forms of that pass that fit the
+// registers (a loop over its six steps) spilled or, written out, measured a
+// seventh slower than the six-field y pass (1024^3 FP64 K2 10.8 -> 12.4 ms
+// against K1 13.4 -> 12.2 ms), so there the z-inverse writes all six fields.
+template <int NY>
+__host__ __device__ constexpr bool curl_in_y() {
+  return NY <= 512;
+}
+
 // O32: every element offset inside one component of uh and w1 fits 32 bits
 // (M and s1 below 2^32); the per-entry offsets then take one register each.
-template <typename T, int N, bool O32>
+template <typename T, int N, bool O32, bool FIVE>
 __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ w1, Geo g, Tables tb,
     const cpx<T>* __restrict__ tw, double* __restrict__ part, long long ntiles) {
@@ -234,11 +247,18 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
     cpx<T>* pen = lds + pl * LD;
     cpx<T>* pen1 = C::ONE ? pen : lds + (B + pl) * LD;
     const long long zs = (long long)g.nyc * g.nhp;
-    // Six output fields from four component loads, each issued at least one
-    // transform before it is needed, with two raw components X, Y held and no
-    // register copies between them:
+    // FIVE (curl_in_y): five output fields from four component loads, each
+    // issued at least one transform before it is needed, with two raw
+    // components X, Y held and no register copies between them.  The x and y
+    // parts of w = curl u^ are constant along a z pencil, so the z-inverse
+    // needs only u and the two kz terms: fields 0..4 = u1, u2, u3,
+    // d1 = F(i a3 u1), d2 = F(i a3 u2), and the y pass forms w1 = i a2 U3 - d2,
+    // w2 = d1 - i a1 U3, w3 = i (a1 U2 - a2 U1).  The enstrophy partials take
+    // w^ itself (a pair of raw components per term):
+    //   u1 | w2 = i(a3 X - a1 Y) | d1, X <- u2 | u3 | w1 = i(a2 Y - a3 X) | d2, Y <- u1 | u2 | w3 = i(a1 X - a2 Y)
+    // starting from X = u1, Y = u3 (the last load only serves the enstrophy).
+    // Otherwise six fields 0..5 = u1, u2, u3, w1, w2, w3 from the same loads:
     //   u1 | w2 = i(a3 X - a1 Y), X <- u2 | u3 | w1 = i(a2 Y - a3 X), Y <- u1 | u2 | w3 = i(a1 X - a2 Y)
-    // starting from X = u1, Y = u3.  Fields are stored as 0..5 = u1, u2, u3, w1, w2, w3.
     // Entries outside the ball load the pencil's kz = 0 entry instead (a valid
     // address; on the disk a line the kept lanes fetch anyway) and are zeroed
     // where they are used.  No branch surrounds a load or a store, so the
@@ -285,8 +305,37 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       }
       emit(v, field);
     };
-    // w = i(ca A - cb B), ia / ib = 1, 2, 3 naming which of a1, a2, a3 ca and cb are;
-    // then the next component (nc >= 0) loads into A
+    // enstrophy of w = i(ca A - cb B), ia / ib = 1, 2, 3 naming which of a1,
+    // a2, a3 ca and cb are
+    auto enstrophy = [&](const cpx<T> (&A)[E], const cpx<T> (&Bv)[E], int ia, int ib) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T a3 = a3l[t + TP * m];
+        const T ca = ia == 1 ? a1 : (ia == 2 ? a2 : a3);
+        const T cb = ib == 1 ? a1 : (ib == 2 ? a2 : a3);
+        const cpx<T> w = keep_or_zero(cmuli(csub(cscale(A[m], ca), cscale(Bv[m], cb))), keep[m]);
+        za += wx * ((double)w.x * w.x + (double)w.y * w.y);
+      }
+    };
+    // d = i a3 A into field, with the enstrophy of w = i(cp P - cq Q) taken in
+    // the same pass over the elements; then component nc (>= 0) loads into L
+    auto dz = [&](const cpx<T> (&A)[E], int field, cpx<T> (&L)[E], int nc, const cpx<T> (&P)[E],
+                  const cpx<T> (&Q)[E], int ip, int iq) {
+      cpx<T> v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const T a3 = a3l[t + TP * m];
+        const T cp = ip == 1 ? a1 : (ip == 2 ? a2 : a3);
+        const T cq = iq == 1 ? a1 : (iq == 2 ? a2 : a3);
+        const cpx<T> w = keep_or_zero(cmuli(csub(cscale(P[m], cp), cscale(Q[m], cq))), keep[m]);
+        za += wx * ((double)w.x * w.x + (double)w.y * w.y);
+        v[m] = keep_or_zero(cmuli(cscale(A[m], a3)), keep[m]);
+      }
+      if (nc >= 0) loadc(L, nc);
+      emit(v, field);
+    };
+    // (six fields) w = i(ca A - cb B) into field, with its enstrophy; then the
+    // next component (nc >= 0) loads into A
     auto vorticity = [&](cpx<T> (&A)[E], const cpx<T> (&Bv)[E], int ia, int ib, int field, int nc) {
       cpx<T> v[E];
 #pragma unroll
@@ -300,12 +349,21 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       if (nc >= 0) loadc(A, nc);
       emit(v, field);
     };
-    // The six transforms written out one after another hold more registers
+    // The transforms written out one after another hold more registers
     // than a loop over them; where that spills (short pencils, where more
     // pencils share a workgroup), a loop over the fields loads the components
-    // of field f + 1 (one or two, nine in all) while field f transforms.
+    // of field f + 1 (one or two, eight in all) while field f transforms.
     constexpr bool UNROLLED = sizeof(T) == 8 ? N >= 256 : N >= 512;
-    if constexpr (UNROLLED) {
+    if constexpr (UNROLLED && FIVE) {
+      loadc(X, 0);
+      loadc(Y, 2);
+      velocity(X, 0);
+      dz(X, 3, X, 1, X, Y, 3, 1);
+      velocity(Y, 2);
+      dz(X, 4, Y, 0, Y, X, 2, 3);
+      velocity(X, 1);
+      enstrophy(X, Y, 1, 2);
+    } else if constexpr (UNROLLED) {
       loadc(X, 0);
       loadc(Y, 2);
       velocity(X, 0);
@@ -314,6 +372,32 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       vorticity(Y, X, 2, 3, 3, 0);
       velocity(X, 1);
       vorticity(X, Y, 1, 2, 5, -1);
+    } else if constexpr (FIVE) {
+      // field f: f = 0, 1, 2 -> u_f (X);  f = 3, 4 -> d = i a3 X with X = u1, u2;
+      // Y pairs with X for the enstrophy terms w1 (f = 1), w2 (f = 3), w3 (f = 4)
+      auto load = [&](int f) {
+        loadc(X, f < 3 ? f : f - 3);
+        if (f == 1 || f >= 3) loadc(Y, f == 4 ? 0 : 2);
+      };
+      load(0);
+#pragma unroll 1
+      for (int f = 0; f < 5; ++f) {
+        if (f == 1) enstrophy(Y, X, 2, 3);
+        if (f == 3) enstrophy(X, Y, 3, 1);
+        if (f == 4) enstrophy(X, Y, 1, 2);
+        cpx<T> v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          if (f < 3) {
+            v[m] = keep_or_zero(X[m], keep[m]);
+            ea += wx * ((double)v[m].x * v[m].x + (double)v[m].y * v[m].y);
+          } else {
+            v[m] = keep_or_zero(cmuli(cscale(X[m], a3l[t + TP * m])), keep[m]);
+          }
+        }
+        if (f + 1 < 5) load(f + 1);
+        emit(v, f);
+      }
     } else {
       // field f: f = 0, 1, 2 -> u_f;  f = 3, 4, 5 -> w = i (ca u_ia - cb u_ib)
       auto load = [&](int f) {
@@ -492,7 +576,8 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_dp_z_inverse(
 
 // ---- K2 / K4: y passes over (z, kx < nhk) pencils ------------------------------
 // DIR = +1 (inverse; NF = 6 for u x omega, 2 for the scalar product): loads
-//   the disk ky from the compact buffer, stores all y to the full-y buffer.
+//   the disk ky from the compact buffer, stores all y to the full-y buffer;
+//   for u x omega the five z-pass fields give the six (w formed here).
 // DIR = -1 (forward; NF = 3, or 1): loads all y from the full-y buffer,
 //   stores the disk ky to the compact buffer.
 // The next field's loads are issued before the current field's transform.
@@ -581,6 +666,77 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_y(const cpx<T>* __restr
       }
     }
   };
+  if constexpr (DIR > 0 && NF == 6 && curl_in_y<N>()) {
+    // u x omega: the z pass wrote 0..4 = U1, U2, U3, D1 = F(i a3 u1), D2 =
+    // F(i a3 u2); the six fields out (0..5 = u1, u2, u3, w1, w2, w3) take w
+    // from them in registers:
+    //   w1 = i a2 U3 - D2,  w2 = D1 - i a1 U3,  w3 = i (a1 U2 - a2 U1)
+    // with two raw fields R1, R2 held, each load issued at least one transform
+    // before its use:  R1 = U3, R2 = D2 | w1, R2 <- D1 | w2, R2 <- U1 | u3,
+    // R1 <- U2 | w3 | u1 | u2.
+    // the ky factors from an LDS copy (held in registers they spilled in
+    // FP32 at Ny = 1024), complete after the barrier below
+    __shared__ T a2l[N];
+    for (int i = threadIdx.x; i < N; i += C::NT) a2l[i] = T(tb.kd2[i]);
+    __syncthreads();
+    const T a1 = T(tb.kd1[kx]);
+    cpx<T> R1[E], R2[E];
+    auto out = [&](cpx<T>(&v)[E], int f) {
+      fast::fft<T, N, E, DIR, 0, C::PD, C::ONE>(v, pen, pen1, t, twp());
+      if (st_ok) {
+        cpx<T>* o = dst + f * g.s2 + fbase;
+#pragma unroll
+        for (int m = 0; m < E; ++m) st_s<NTS>(o + (long long)(t + TP * m) * g.nhp, v[m]);
+      }
+      if constexpr (STREAM) {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    // STREAM: entries not held were loaded from the pencil's first entry
+    auto held_only = [&](cpx<T> x, int m) { return STREAM ? keep_or_zero(x, held(m)) : x; };
+    load(2, R1);
+    load(4, R2);
+    // step f: the field formed, the load issued before its transform, the
+    // output field (the steps written out)
+    auto curl_step = [&](int f) {
+      cpx<T> v[E];
+      int of;
+      if (f == 0) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = held_only(csub(cmuli(cscale(R1[m], a2l[t + TP * m])), R2[m]), m);
+        load(3, R2);
+        of = 3;
+      } else if (f == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = held_only(csub(R2[m], cmuli(cscale(R1[m], a1))), m);
+        load(0, R2);
+        of = 4;
+      } else if (f == 2) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = held_only(R1[m], m);
+        load(1, R1);
+        of = 2;
+      } else if (f == 3) {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          v[m] = held_only(cmuli(csub(cscale(R1[m], a1), cscale(R2[m], a2l[t + TP * m]))), m);
+        of = 5;
+      } else if (f == 4) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = held_only(R2[m], m);
+        of = 0;
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = held_only(R1[m], m);
+        of = 1;
+      }
+      out(v, of);
+    };
+#pragma unroll
+    for (int f = 0; f < 6; ++f) curl_step(f);
+    return;
+  }
   auto step = [&](int f) {
     cpx<T> v[E];
 #pragma unroll
@@ -1154,15 +1310,27 @@ static unsigned launch_z_inverse(const gbg_plan* p, const Geo& g, const cpx<T>* 
                                  const Tables& tb, const cpx<T>* twz, double* part, hipStream_t s) {
   using C = Cfg<T, N>;
   const long long nt = ((long long)g.nyc * g.nhp + C::B - 1) / C::B;
+  // five fields where the y pass forms the kx and ky terms of w (curl_in_y
+  // of Ny, a power of two here); six otherwise
+  const bool five = g.ny <= 512;
+  static_assert(curl_in_y<512>() && !curl_in_y<1024>(), "five-field threshold");
   unsigned grid;
   if (narrow_offsets(p, g)) {
     grid = pass_grid(p, nt, z_inverse_persist<T, N, true>());
-    hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
-                       tb, twz, part, nt);
+    if (five)
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true, true>), dim3(grid), dim3(C::NT), 0, s, uh, W1,
+                         g, tb, twz, part, nt);
+    else
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, true, false>), dim3(grid), dim3(C::NT), 0, s, uh, W1,
+                         g, tb, twz, part, nt);
   } else {
     grid = pass_grid(p, nt, z_inverse_persist<T, N, false>());
-    hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false>), dim3(grid), dim3(C::NT), 0, s, uh, W1, g,
-                       tb, twz, part, nt);
+    if (five)
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false, true>), dim3(grid), dim3(C::NT), 0, s, uh,
+                         W1, g, tb, twz, part, nt);
+    else
+      hipLaunchKernelGGL((k_uxw_z_inverse<T, N, false, false>), dim3(grid), dim3(C::NT), 0, s, uh,
+                         W1, g, tb, twz, part, nt);
   }
   return grid;
 }
