@@ -236,7 +236,10 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
       }
     }
     if (!__syncthreads_or(disk)) return;
-    double ea = 0.0, za = 0.0;   // this tile's sums (local: no pointer to the outer ones)
+    // the sums accumulate straight into the workgroup's (a per-tile copy cost
+    // the FP32 K1 at Nz = 512 four registers it spilled inside the transforms)
+    double& ea = eacc;
+    double& za = zacc;
     const T a1 = PREF ? q.a1 : T(tb.kd1[kx]), a2 = PREF ? q.a2 : T(tb.kd2[ky]);
     const double wx = (kx == 0 || kx == g.nyq_x) ? 1.0 : 2.0;
     const long long plane = (long long)g.ny * g.nh;
@@ -425,8 +428,6 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
         emit(v, f);
       }
     }
-    eacc += ea;
-    zacc += za;
   };
   constexpr bool PERSIST = z_inverse_persist<T, N, O32>();
   if constexpr (PERSIST) {
