@@ -120,6 +120,20 @@ constexpr bool z_inverse_persist() {
   return sizeof(T) == 8 ? (N >= 128 && N <= 512) : (O32 && N == 512);
 }
 
+// Nontemporal stores in the z-inverse pass only where a tile's piece of a z
+// plane (B pencils) fills 128-byte lines: a 64-byte piece (B = 4 in FP64, 8 in
+// FP32, at Nz = 1024) shares its line with the next tile's, and with the hint
+// the two halves can reach memory as partial lines instead of meeting in L2 (a
+// probe of that pattern: 64-byte pieces 3.3-3.8 TB/s written with the hint,
+// 128-byte pieces 5.0-6.0 TB/s).  Measured at 1024^3: K1 with plain stores
+// 13.36 -> 13.22 ms FP64 (zero tiles beside it), 7.40 -> 7.23 ms FP32; the
+// last pass's transform kernel, whose output tiles follow the tile list, was
+// slower with plain stores (6.07 -> 6.49 ms FP64), so it keeps the hint.
+template <typename T, int B>
+constexpr bool z_store_nt() {
+  return (GBG_NT & 2) != 0 && B * (int)sizeof(cpx<T>) >= 128;
+}
+
 // Where the y-inverse pass forms the kx and ky terms of w = curl u^ (the
 // z-inverse then writes five fields: u and the two kz terms of w): Ny <= 512,
 // where the y passes stream.  At Ny = 1024 the forms of that pass that fit the
@@ -140,6 +154,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
   using C = Cfg<T, N>;
   using Off = typename std::conditional<O32, unsigned, long long>::type;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
+  constexpr bool NTS = z_store_nt<T, B>();
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   __shared__ double red[2][C::NT / 64];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
@@ -293,7 +308,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_inverse(
         for (int m = 0; m < E; ++m) {
           Off zo = (Off)p1 + (Off)(t + TP * m) * (Off)zs;
           asm volatile("" : "+v"(zo));
-          st_s(o + zo, v[m]);
+          st_s<NTS>(o + zo, v[m]);
         }
       }
       asm volatile("" ::: "memory");
@@ -1040,6 +1055,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
     const cpx<T>* __restrict__ tw, const int* __restrict__ list, long long ntiles, Epi<T> ep) {
   using C = Cfg<T, N>;
   constexpr int E = C::E, TP = C::TP, B = C::B, LD = C::LD;
+  constexpr bool NTS = (GBG_NT & 2) != 0;
   __shared__ cpx<T> lds[C::NBUF * B * LD];
   const int pl = threadIdx.x % B, t = threadIdx.x / B;
   // A global load consumed between the component loads and their last use
@@ -1111,7 +1127,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
               if (ep.store_n) nh[co] = mk<T>(T(0), T(0));
               epi_store(ep, co, qxy + q3(t + TP * m), mk<T>(T(0), T(0)));
             } else {
-              st_s(nh + co, mk<T>(T(0), T(0)));
+              st_s<NTS>(nh + co, mk<T>(T(0), T(0)));
             }
           }
       }
@@ -1159,7 +1175,7 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const long long o = (long long)(t + TP * m) * plane + p5;
-        st_s(nh + o, mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0)));
+        st_s<NTS>(nh + o, mk<T>(keep[m] ? n[0][m].x : T(0), keep[m] ? n[0][m].y : T(0)));
       }
     } else {
 #pragma unroll
@@ -1191,9 +1207,9 @@ __global__ __launch_bounds__((Cfg<T, N>::NT)) void k_uxw_z_forward(
           epi_store(ep, g.M + o, q, r2);
           epi_store(ep, 2 * g.M + o, q, r3);
         } else {
-          st_s(nh + o, r1);
-          st_s(nh + g.M + o, r2);
-          st_s(nh + 2 * g.M + o, r3);
+          st_s<NTS>(nh + o, r1);
+          st_s<NTS>(nh + g.M + o, r2);
+          st_s<NTS>(nh + 2 * g.M + o, r3);
         }
       }
     }
