@@ -131,7 +131,7 @@ constexpr bool z_inverse_persist() {
 // slower with plain stores (6.07 -> 6.49 ms FP64), so it keeps the hint.
 template <typename T, int B>
 constexpr bool z_store_nt() {
-  return (GBG_NT & 2) != 0 && B * (int)sizeof(cpx<T>) >= 128;
+  return piece_store_nt(B * (int)sizeof(cpx<T>));
 }
 
 // Where the y-inverse pass forms the kx and ky terms of w = curl u^ (the
