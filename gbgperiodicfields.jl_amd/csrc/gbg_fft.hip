@@ -323,7 +323,7 @@ __global__ __launch_bounds__((Cfg<T, N>::XNT)) void k_fast_r2c_x(const T* __rest
   cpx<T>* pen = lds + un * LD;
   cpx<T>* pen1 = lds + (C::U + un) * LD;
   fast::fft<T, N, E, -1, C::XSYNC, C::XPD>(v, pen, pen1, t, tw);
-  fast::r2c_pair_store<T, N, E>(v, pen, out + (ok0 ? r0 * g.nh : 0), out + (ok1 ? r1 * g.nh : 0),
+  fast::r2c_pair_store<T, N, E, RowG, false>(v, pen, out + (ok0 ? r0 * g.nh : 0), out + (ok1 ? r1 * g.nh : 0),
                                 ok0, ok1, t, g);
 }
 
