@@ -248,3 +248,32 @@ def test_closed_forms_pinned_by_oracle():
     got, E, Z = O.uxw(dense(abc_modes(512.0, A, B, Cc), n), g, O.dealias_cutoff(1024))
     assert np.abs(got).max() < 1e-12
     assert abs(E - (A * A + B * B + Cc * Cc) / 2) < 1e-14 and abs(Z - E) < 1e-14
+
+
+def test_curl_formed_after_the_z_transform():
+    """The fused u x omega schedule's split of w = curl u^: the z-inverse pass
+    writes U1, U2, U3 and the kz terms D1 = Fz(i a3 u1), D2 = Fz(i a3 u2), and
+    the y pass forms w1 = i a2 U3 - D2, w2 = D1 - i a1 U3, w3 = i (a1 U2 - a2 U1)
+    (a1, a2 are constant along a z pencil, Fz is linear).  On a dense random
+    spectrum of an anisotropic grid that equals the z transform of the
+    oracle's curl, and a sign or factor slip in any of the three fails it."""
+    n = (12, 10, 16)
+    g = O.Grid3D(n)
+    m = O.Multipliers.build(g)
+    rng = np.random.default_rng(7)
+    shape = (3,) + g.spectral_shape
+    u = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+    def fz(f):    # unnormalised inverse transform along z (axis 0 of a component)
+        return np.fft.ifft(f, axis=0) * n[2]
+
+    ref = np.stack([fz(c) for c in O.curl(u, m)])
+    a1, a2, a3 = m.a1(), m.a2(), m.a3()
+    U = [fz(c) for c in u]
+    d1, d2 = fz(1j * a3 * u[0]), fz(1j * a3 * u[1])
+    w = np.stack([1j * a2 * U[2] - d2, d1 - 1j * a1 * U[2], 1j * (a1 * U[1] - a2 * U[0])])
+    scale = np.abs(ref).max()
+    assert np.abs(w - ref).max() <= 1e-13 * scale
+    # the same split with D2's sign flipped is caught
+    bad = 1j * a2 * U[2] + d2
+    assert np.abs(bad - ref[0]).max() > 1e-3 * scale
