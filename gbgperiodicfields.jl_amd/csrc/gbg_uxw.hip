@@ -1661,8 +1661,10 @@ struct SlabCfg {
 // 2.86 -> 2.82 ms (16 planes), 1024^3 FP32 30.0 -> 30.2 ms (8 planes; 4 and
 // 2 planes slower still); on smaller grids the short per-slab launches cannot
 // fill the device.  Three slots in flight, and one stream per pass with the
-// slabs flowing through them under events, measured slower.  What the chain
-// saves is memory: the six full-y fields (6 Nz Ny nhp entries, 4.4 GB at
+// slabs flowing through them under events, measured slower.  Since the passes
+// store with the nontemporal hint (and the z pass writes five fields) the chain
+// is slower at 512^3 too: 4.93 -> 5.06 ms FP64, 2.65 -> 2.71 ms FP32 (medians
+// of interleaved runs).  What the chain saves is memory: the six full-y fields (6 Nz Ny nhp entries, 4.4 GB at
 // 512^3 FP64, 35 GB at 1024^3) become two slots.  GBG_UXW_SLAB=Z,R at plan
 // creation sets the planes per slab and the slots for such calls (tests and
 // tuning); the results are the same bits for every setting.
