@@ -179,6 +179,27 @@ static gbg_status sc_uxw_slabs(int elem) {
 }
 static gbg_status sc_uxw_slabs64() { return sc_uxw_slabs(8); }
 static gbg_status sc_uxw_slabs32() { return sc_uxw_slabs(4); }
+// the fused u x omega at Ny = 1024, where the z pass writes all six fields
+// (the y pass forms the kx and ky terms of w only up to Ny = 512)
+static gbg_status sc_uxw_ny1024(int elem) {
+  const int n[3] = {32, 1024, 32};
+  const double L[3] = {6.283185307179586, 6.283185307179586, 6.283185307179586};
+  Plan pl;
+  TRY(gbg_plan_create(n, L, elem, &pl.h));
+  size_t M = 0, P = 0, dev = 0;
+  TRY(gbg_plan_query(pl.h, &M, &P, &dev));
+  const size_t sc = M * 2 * elem;
+  Bufs b;
+  void *spec3, *out3;
+  TRY(b.get(&spec3, 3 * sc));
+  TRY(b.get(&out3, 3 * sc));
+  TRY(gbg_memset(spec3, 0, 3 * sc, nullptr));
+  double E = 0, Z = 0;
+  TRY(gbg_uxw(pl.h, spec3, out3, 12.0, &E, &Z, 0, nullptr));
+  return GBG_OK;
+}
+static gbg_status sc_uxw_ny1024_64() { return sc_uxw_ny1024(8); }
+static gbg_status sc_uxw_ny1024_32() { return sc_uxw_ny1024(4); }
 static gbg_status sc_ops_composed() {
   const int n[3] = {16, 12, 10};
   return ops_on(n, 8, nullptr);
@@ -329,6 +350,8 @@ int main() {
              {"ops fused FP32 64x32x32", sc_ops_fused32},
              {"uxw slabs FP64 32x32x64", sc_uxw_slabs64},
              {"uxw slabs FP32 32x32x64", sc_uxw_slabs32},
+             {"uxw six fields FP64 32x1024x32", sc_uxw_ny1024_64},
+             {"uxw six fields FP32 32x1024x32", sc_uxw_ny1024_32},
              {"ops composed 16x12x10", sc_ops_composed},
              {"ops layout (3,2,1) 8x7x6", sc_ops_layout},
              {"ops chirp-z FP32 17x19x6", sc_ops_chirp},
